@@ -1,0 +1,196 @@
+/*
+ * slimemold.h - C ABI of the MI355X slime-mold step (libslimemold_hip.so).
+ *
+ * This is the seam that replaces the reference's wgpu compute dispatch: the
+ * three compute pipelines built from src/compute.wgsl
+ * (src/pipeline_manager.rs:37-54), their bind group (agents rw, trail rw,
+ * uniform; src/pipeline_manager.rs:77-113, src/bind_group_manager.rs:50-65,
+ * :101-126) and the host code in src/main.rs that uploads parameters and
+ * dispatches, clears, resizes and re-initialises the simulation.
+ *
+ * Conventions
+ *   - Every function returns SLM_OK (0) or a negative SLM_ERR_* code and never
+ *     aborts or throws.  slm_last_error() gives a message for the calling
+ *     thread's most recent failure.
+ *   - A simulation handle owns all of its device memory and one HIP stream.
+ *     It may be used by one host thread at a time; distinct handles are
+ *     independent.  slm_step only enqueues work; the read_* and total_mass
+ *     calls wait for it.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - Agents are exchanged as the reference lays them out: n records of four
+ *     f32 (x, y, angle, speed) (src/compute.wgsl:2,31-32), in the original
+ *     agent order whatever re-sorting the library does internally.
+ *   - The trail is exchanged as row-major f32, index y*width + x
+ *     (src/compute.wgsl:34-35), whatever the storage format.
+ *
+ * Semantics: the per-step update is the reference's (src/compute.wgsl:57-195)
+ * made race-free: every agent moves exactly once per step; sensing reads the
+ * trail as it was when the step began; a cell that agents land in becomes
+ * clamp(T + deposit, 0, 1) (the reference's result for deposit >= 1, i.e. for
+ * every preset); decay then a double-buffered toroidal 3x3 box diffuse.
+ */
+#ifndef SLIMEMOLD_H
+#define SLIMEMOLD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLM_ABI_VERSION 1
+
+/* ---- status codes --------------------------------------------------------- */
+#define SLM_OK 0
+#define SLM_ERR_INVALID_ARGUMENT (-1)
+#define SLM_ERR_OUT_OF_MEMORY (-2)
+#define SLM_ERR_HIP (-3)
+#define SLM_ERR_NO_DEVICE (-4)
+#define SLM_ERR_UNSUPPORTED (-5)
+
+/* ---- parameters ----------------------------------------------------------
+ * The fields of the reference's SimSizeUniform after width and height, in the
+ * same order (src/main.rs:29-46, built from Settings by src/main.rs:48-67).
+ * decay_factor is the raw setting (e.g. 10.0); the step subtracts
+ * decay_factor * 0.001 (src/compute.wgsl:159).  blur_radius and blur_sigma
+ * are carried for layout compatibility; the reference shader never reads them
+ * (src/compute.wgsl:49) and neither does the 3x3 box diffuse. */
+typedef struct slm_params {
+  float decay_factor;
+  float agent_jitter;
+  float agent_speed_min;
+  float agent_speed_max;
+  float agent_turn_speed;
+  float agent_sensor_angle;
+  float agent_sensor_distance;
+  float diffusion_rate;
+  float pheromone_deposition_amount;
+  float blur_radius;
+  float blur_sigma;
+} slm_params;
+
+/* Trail storage in HBM.  Deposit, decay and diffuse arithmetic is f32 in all
+ * three; the 16-bit formats round the stored result to nearest-even. */
+typedef enum slm_storage {
+  SLM_STORAGE_F32 = 0,
+  SLM_STORAGE_BF16 = 1,
+  SLM_STORAGE_F16 = 2
+} slm_storage;
+
+/* Creation-time configuration.  Start from slm_config_default(). */
+typedef struct slm_config {
+  uint32_t width;           /* grid width in cells  (>= 1) */
+  uint32_t height;          /* grid height in cells (>= 1); width*height <= 2^31 */
+  uint64_t agent_count;     /* >= 0, < 2^32 */
+  uint64_t seed;            /* agent initialisation seed (SplitMix64) */
+  int32_t storage;          /* slm_storage */
+  int32_t device;           /* HIP device ordinal */
+  uint32_t resort_interval; /* Morton re-sort of agents every k steps; 0 = never */
+  uint32_t graph_chunk;     /* steps per captured hipGraph; 0 = launch eagerly */
+  uint32_t sort_cell_log2;  /* re-sort bin edge = 2^k cells (default 4) */
+  uint32_t flags;           /* SLM_FLAG_* */
+  uint32_t reserved[6];     /* must be zero */
+} slm_config;
+
+/* Map consecutive workgroups of a launch onto one XCD's L2 (default on). */
+#define SLM_FLAG_NO_XCD_REMAP 0x1u
+
+typedef struct slm_info {
+  uint32_t width, height;
+  uint64_t agent_count;
+  int32_t storage;
+  int32_t device;
+  uint64_t steps_done;
+  uint64_t device_bytes; /* device memory held by the handle */
+  uint32_t resort_interval, graph_chunk;
+} slm_info;
+
+/* Per-kernel device time (HIP events on the handle's stream). */
+typedef struct slm_timing {
+  double agents_ms;   /* total over the call */
+  double trail_ms;    /* fused deposit + decay + diffuse */
+  double sort_ms;     /* Morton re-sorts */
+  uint64_t agents_launches, trail_launches, sort_launches;
+} slm_timing;
+
+typedef struct slm_sim slm_sim;
+
+/* Thread-local message for the most recent failing call (never NULL). */
+const char* slm_last_error(void);
+
+/* Number of HIP devices visible (0 when none). */
+int slm_device_count(int* out_count);
+
+/* Fill *cfg with defaults: 1600x900 (src/settings.rs:4-5), 0 agents, seed 0,
+ * f32 storage, device 0, no re-sort, 100-step graphs, 16-cell sort bins. */
+int slm_config_default(slm_config* cfg);
+
+/* Fill *params with the reference's defaults (src/settings.rs:9-27). */
+int slm_params_default(slm_params* params);
+
+/* Allocate a simulation: agents initialised on the device from cfg->seed as
+ * src/main.rs:262-282 does on the host; trail zero (src/main.rs:286-293,
+ * wgpu zero-initialises new buffers).  Replaces the buffer and pipeline
+ * creation of src/main.rs:263-323 and PipelineManager::new
+ * (src/pipeline_manager.rs:20-75). */
+int slm_create(const slm_config* cfg, const slm_params* params, slm_sim** out);
+
+/* Release everything the handle owns (waits for its stream). */
+int slm_destroy(slm_sim* sim);
+
+/* Replace the parameters for subsequent steps; agents and trail persist.
+ * Replaces update_settings / the preset switch (src/main.rs:83-99, :815-832).
+ * Captured graphs stay valid: kernels read parameters from device memory. */
+int slm_set_params(slm_sim* sim, const slm_params* params);
+int slm_get_params(const slm_sim* sim, slm_params* out);
+
+/* Enqueue n steps: agents (src/compute.wgsl:57-145) then the fused deposit +
+ * decay + diffuse pass (:147-195).  Replaces the dispatch block
+ * src/main.rs:1163-1235 (the display pass there only reads the trail). */
+int slm_step(slm_sim* sim, uint32_t n_steps);
+
+/* As slm_step, launched eagerly with HIP events around every kernel; waits
+ * and adds the per-kernel device times to *out (zeroed first). */
+int slm_step_timed(slm_sim* sim, uint32_t n_steps, slm_timing* out);
+
+/* Wait for all work enqueued on the handle's stream. */
+int slm_synchronize(slm_sim* sim);
+
+/* Zero the trail (src/main.rs:908-913). */
+int slm_clear(slm_sim* sim);
+
+/* New grid size: x *= new_w/old_w and y *= new_h/old_h in f32, trail
+ * reallocated and zeroed (src/main.rs:921-1085).  Unlike the reference the
+ * current parameters are kept (it re-uploads the start-up decay factor). */
+int slm_resize(slm_sim* sim, uint32_t width, uint32_t height);
+
+/* New agent count: every agent re-initialised from the seed, as the N key
+ * does (src/main.rs:682-791), using the current speed range. */
+int slm_set_agent_count(slm_sim* sim, uint64_t agent_count);
+
+/* speed = min + u*(max-min) for every agent, u from a fresh generator stream
+ * per call (src/main.rs:101-145). */
+int slm_reassign_speeds(slm_sim* sim, float speed_min, float speed_max);
+
+/* Copy agents out/in as n (x, y, angle, speed) f32 records; n must equal the
+ * agent count.  Written positions must be finite and inside [0,width] x
+ * [0,height]; angle and speed must be finite. */
+int slm_read_agents(slm_sim* sim, float* agents_xyzw, uint64_t n);
+int slm_write_agents(slm_sim* sim, const float* agents_xyzw, uint64_t n);
+
+/* Copy the trail out/in as width*height f32.  Written values must be finite
+ * and >= 0 (-0 is stored as +0); 16-bit storage rounds them to nearest-even. */
+int slm_read_trail(slm_sim* sim, float* trail, uint64_t cells);
+int slm_write_trail(slm_sim* sim, const float* trail, uint64_t cells);
+
+/* Sum of the trail, accumulated in f64 on the device in a fixed order. */
+int slm_total_mass(slm_sim* sim, double* out);
+
+int slm_get_info(const slm_sim* sim, slm_info* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLIMEMOLD_H */

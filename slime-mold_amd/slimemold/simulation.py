@@ -1,0 +1,156 @@
+"""Simulation: the host-side handle over the C ABI.
+
+It carries the operations the reference performs on its GPU buffers:
+step (src/main.rs:1163-1235), clear (:908-913), resize (:921-1085), agent
+count change (:682-791), speed reassignment (:101-145) and parameter upload
+(:83-99, the preset switch :815-832).  Agents are exchanged as an (n, 4)
+float32 array of (x, y, angle, speed) rows, the trail as an (height, width)
+float32 array.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Union
+
+import numpy as np
+
+from . import _native as N
+from .settings import Settings
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _params(p: Union[Settings, "N.Params", None]) -> "N.Params":
+    if p is None:
+        return N.default_params()
+    if isinstance(p, Settings):
+        return p.to_params()
+    return p
+
+
+class Simulation:
+    def __init__(self, width: int, height: int, agent_count: int, seed: int = 0,
+                 params: Union[Settings, "N.Params", None] = None, storage: str = "f32",
+                 resort_interval: int = 0, graph_chunk: int = 100, sort_cell_log2: int = 4,
+                 device: int = 0, xcd_remap: bool = True):
+        cfg = N.default_config()
+        cfg.width = width
+        cfg.height = height
+        cfg.agent_count = agent_count
+        cfg.seed = seed
+        cfg.storage = N.STORAGE_NAMES[storage]
+        cfg.device = device
+        cfg.resort_interval = resort_interval
+        cfg.graph_chunk = graph_chunk
+        cfg.sort_cell_log2 = sort_cell_log2
+        cfg.flags = 0 if xcd_remap else N.FLAG_NO_XCD_REMAP
+        self._params = _params(params)
+        h = ctypes.c_void_p()
+        N.check(N.lib().slm_create(ctypes.byref(cfg), ctypes.byref(self._params), ctypes.byref(h)))
+        self._h: Optional[ctypes.c_void_p] = h
+        self.storage = storage
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if self._h is not None:
+            N.check(N.lib().slm_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise N.SlimeMoldError(N.SLM_ERR_INVALID_ARGUMENT, "simulation is closed")
+        return self._h
+
+    # -- state --------------------------------------------------------------
+    def info(self) -> "N.Info":
+        i = N.Info()
+        N.check(N.lib().slm_get_info(self.handle, ctypes.byref(i)))
+        return i
+
+    @property
+    def width(self) -> int:
+        return self.info().width
+
+    @property
+    def height(self) -> int:
+        return self.info().height
+
+    @property
+    def agent_count(self) -> int:
+        return self.info().agent_count
+
+    def set_params(self, params: Union[Settings, "N.Params"]) -> None:
+        p = _params(params)
+        N.check(N.lib().slm_set_params(self.handle, ctypes.byref(p)))
+        self._params = p
+
+    def get_params(self) -> "N.Params":
+        p = N.Params()
+        N.check(N.lib().slm_get_params(self.handle, ctypes.byref(p)))
+        return p
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, n: int = 1) -> None:
+        N.check(N.lib().slm_step(self.handle, n))
+
+    def step_timed(self, n: int) -> "N.Timing":
+        t = N.Timing()
+        N.check(N.lib().slm_step_timed(self.handle, n, ctypes.byref(t)))
+        return t
+
+    def synchronize(self) -> None:
+        N.check(N.lib().slm_synchronize(self.handle))
+
+    # -- edits --------------------------------------------------------------
+    def clear(self) -> None:
+        N.check(N.lib().slm_clear(self.handle))
+
+    def resize(self, width: int, height: int) -> None:
+        N.check(N.lib().slm_resize(self.handle, width, height))
+
+    def set_agent_count(self, n: int) -> None:
+        N.check(N.lib().slm_set_agent_count(self.handle, n))
+
+    def reassign_speeds(self, speed_min: float, speed_max: float) -> None:
+        N.check(N.lib().slm_reassign_speeds(self.handle, speed_min, speed_max))
+
+    # -- transfers ------------------------------------------------------------
+    def read_agents(self) -> np.ndarray:
+        n = self.agent_count
+        out = np.empty((n, 4), dtype=np.float32)
+        N.check(N.lib().slm_read_agents(self.handle, _fptr(out), n))
+        return out
+
+    def write_agents(self, agents: np.ndarray) -> None:
+        a = np.ascontiguousarray(agents, dtype=np.float32).reshape(-1, 4)
+        N.check(N.lib().slm_write_agents(self.handle, _fptr(a), a.shape[0]))
+
+    def read_trail(self) -> np.ndarray:
+        i = self.info()
+        out = np.empty((i.height, i.width), dtype=np.float32)
+        N.check(N.lib().slm_read_trail(self.handle, _fptr(out), out.size))
+        return out
+
+    def write_trail(self, trail: np.ndarray) -> None:
+        t = np.ascontiguousarray(trail, dtype=np.float32)
+        N.check(N.lib().slm_write_trail(self.handle, _fptr(t), t.size))
+
+    def total_mass(self) -> float:
+        m = ctypes.c_double(0.0)
+        N.check(N.lib().slm_total_mass(self.handle, ctypes.byref(m)))
+        return m.value

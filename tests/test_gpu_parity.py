@@ -1,0 +1,188 @@
+"""HIP path vs oracle, bit for bit, through the C ABI (ctypes).
+
+The north-star bar is per-agent position/heading within 1e-6 relative and the
+trail within 1e-5 absolute for the first 16 steps; both sides evaluate the same
+f32 operations in the same order (including sin/cos from slm_trig.h), so the
+tests demand exact equality, which is stricter.
+"""
+import numpy as np
+import pytest
+
+from parity_util import PRESET_NAMES, assert_bitwise, preset, run_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def test_init_matches_oracle(slm, orc):
+    s = preset(slm, "Default")
+    for (w, h, n, seed) in [(1024, 1024, 1 << 18, 42), (37, 23, 1000, 7), (1600, 900, 5000, 0)]:
+        with slm.Simulation(w, h, n, seed=seed, params=s) as sim:
+            got = sim.read_agents()
+            assert np.all(sim.read_trail() == 0.0)
+        want = orc.init_agents(seed, w, h, n, s.agent_speed_min, s.agent_speed_max)
+        assert_bitwise(got, want, f"init {w}x{h} n={n}")
+
+
+def test_default_preset_c0_shape_16_steps(slm, orc):
+    """1024x1024, 2^18 agents, Default preset, seed 42: the small headless case."""
+    ga, gt, oa, ot = run_parity(slm, orc, 1024, 1024, 1 << 18, 42, preset(slm, "Default"), 16)
+    assert_bitwise(ga, oa, "agents")
+    assert_bitwise(gt, ot, "trail")
+
+
+@pytest.mark.parametrize("name", PRESET_NAMES)
+def test_every_preset_16_steps(slm, orc, name):
+    ga, gt, oa, ot = run_parity(slm, orc, 256, 256, 1 << 14, 3, preset(slm, name), 16)
+    assert_bitwise(ga, oa, f"{name} agents")
+    assert_bitwise(gt, ot, f"{name} trail")
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("name", ["Default", "Curls"])
+def test_16bit_storage_16_steps(slm, orc, storage, name):
+    ga, gt, oa, ot = run_parity(slm, orc, 512, 384, 1 << 15, 5, preset(slm, name), 16,
+                                storage=storage)
+    assert_bitwise(ga, oa, f"{storage} agents")
+    assert_bitwise(gt, ot, f"{storage} trail")
+
+
+@pytest.mark.parametrize("w,h", [(37, 23), (1, 1), (1, 17), (17, 1), (2, 3), (1028, 60),
+                                 (1030, 40), (1032, 50), (1600, 900), (260, 2)])
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_ragged_grids(slm, orc, w, h, storage):
+    s = preset(slm, "Waves")
+    n = min(4000, w * h * 2)
+    ga, gt, oa, ot = run_parity(slm, orc, w, h, n, 11, s, 12, storage=storage)
+    assert_bitwise(ga, oa, f"{w}x{h} agents")
+    assert_bitwise(gt, ot, f"{w}x{h} trail")
+
+
+def test_random_state_start(slm, orc):
+    rng = np.random.default_rng(12)
+    w, h, n = 300, 200, 20000
+    agents = np.stack([rng.uniform(0, w, n), rng.uniform(0, h, n), rng.uniform(-10, 20, n),
+                       rng.uniform(0, 90, n)], axis=1).astype(np.float32)
+    agents[:5, 0] = w  # on the far edges
+    agents[5:10, 1] = h
+    trail = rng.uniform(0, 1.2, (h, w)).astype(np.float32)
+    s = preset(slm, "Mesh")
+    ga, gt, oa, ot = run_parity(slm, orc, w, h, n, 0, s, 5, trail0=trail, agents0=agents)
+    assert_bitwise(ga, oa, "agents")
+    assert_bitwise(gt, ot, "trail")
+
+
+def test_zero_and_one_agent(slm, orc):
+    rng = np.random.default_rng(13)
+    trail = rng.uniform(0, 1, (64, 64)).astype(np.float32)
+    for n in (0, 1):
+        ga, gt, oa, ot = run_parity(slm, orc, 64, 64, n, 1, preset(slm, "Default"), 7,
+                                    trail0=trail)
+        assert_bitwise(ga, oa, f"n={n} agents")
+        assert_bitwise(gt, ot, f"n={n} trail")
+
+
+@pytest.mark.parametrize("interval", [1, 3])
+def test_resort_gives_identical_results(slm, orc, interval):
+    """Morton re-sort permutes agents internally; outputs are in original order
+    and the result must not change."""
+    s = preset(slm, "Firecracker Trees")
+    ga, gt, oa, ot = run_parity(slm, orc, 512, 512, 1 << 16, 9, s, 20, resort_interval=interval,
+                                graph_chunk=0)
+    assert_bitwise(ga, oa, "agents")
+    assert_bitwise(gt, ot, "trail")
+
+
+def test_graphs_match_eager(slm):
+    s = preset(slm, "Snake")
+    outs = []
+    for chunk, resort in [(0, 0), (10, 0), (10, 4), (7, 3)]:
+        with slm.Simulation(384, 256, 30000, seed=4, params=s, graph_chunk=chunk,
+                            resort_interval=resort) as sim:
+            sim.step(33)
+            sim.step(10)
+            outs.append((sim.read_agents(), sim.read_trail()))
+    for a, t in outs[1:]:
+        assert_bitwise(a, outs[0][0], "agents")
+        assert_bitwise(t, outs[0][1], "trail")
+
+
+def test_graph_replays_track_parameter_changes(slm, orc):
+    """The preset switch (src/main.rs:815-832) lands in captured graphs."""
+    names = ["Default", "Sponge", "Firecracker Trees"]
+    w, h, n = 256, 256, 1 << 14
+    with slm.Simulation(w, h, n, seed=1, params=preset(slm, names[0]), graph_chunk=5) as sim:
+        a = sim.read_agents()
+        t = sim.read_trail()
+        for nm in names:
+            sim.set_params(preset(slm, nm))
+            sim.step(10)
+            orc.step(a, t, preset(slm, nm), 10)
+        assert_bitwise(sim.read_agents(), a, "agents")
+        assert_bitwise(sim.read_trail(), t, "trail")
+
+
+def test_clear_resize_count_speeds(slm, orc):
+    s = preset(slm, "Default")
+    with slm.Simulation(200, 100, 5000, seed=21, params=s) as sim:
+        sim.step(5)
+        assert sim.read_trail().max() > 0
+        sim.clear()
+        assert np.all(sim.read_trail() == 0.0)
+        # resize: positions scaled by the f32 ratio, trail zero (src/main.rs:993-1015)
+        before = sim.read_agents()
+        sim.resize(300, 150)
+        after = sim.read_agents()
+        sx = np.float32(np.float32(300) / np.float32(200))
+        sy = np.float32(np.float32(150) / np.float32(100))
+        assert np.array_equal(after[:, 0], (before[:, 0] * sx).astype(np.float32))
+        assert np.array_equal(after[:, 1], (before[:, 1] * sy).astype(np.float32))
+        assert np.array_equal(after[:, 2:], before[:, 2:])
+        t = sim.read_trail()
+        assert t.shape == (150, 300) and np.all(t == 0.0)
+        sim.step(4)
+        orc.step(after, t, s, 4)
+        assert_bitwise(sim.read_agents(), after, "after resize")
+        # reassign speeds (src/main.rs:101-145): one fresh stream per call
+        a = sim.read_agents()
+        sim.reassign_speeds(10.0, 20.0)
+        orc.reassign_speeds(a, 21, 1, 10.0, 20.0)
+        assert_bitwise(sim.read_agents(), a, "reassign")
+        # agent count: everything re-initialised from the seed (src/main.rs:682-791)
+        sim.set_agent_count(7000)
+        want = orc.init_agents(21, 300, 150, 7000, s.agent_speed_min, s.agent_speed_max)
+        assert_bitwise(sim.read_agents(), want, "set_agent_count")
+
+
+def test_total_mass_f64(slm):
+    rng = np.random.default_rng(14)
+    t = rng.uniform(0, 1, (333, 517)).astype(np.float32)
+    for storage in ("f32", "bf16", "f16"):
+        with slm.Simulation(517, 333, 0, params=preset(slm, "Default"), storage=storage) as sim:
+            sim.write_trail(t)
+            stored = sim.read_trail().astype(np.float64)
+            assert sim.total_mass() == pytest.approx(stored.sum(), rel=1e-13)
+
+
+def test_rejects_bad_input_without_launching(slm):
+    with slm.Simulation(64, 64, 10, params=preset(slm, "Default")) as sim:
+        a = sim.read_agents()
+        bad = a.copy()
+        bad[3, 0] = 65.0  # outside [0, width]
+        with pytest.raises(slm.SlimeMoldError):
+            sim.write_agents(bad)
+        bad = a.copy()
+        bad[0, 2] = np.nan
+        with pytest.raises(slm.SlimeMoldError):
+            sim.write_agents(bad)
+        t = np.zeros((64, 64), np.float32)
+        t[1, 1] = -0.5
+        with pytest.raises(slm.SlimeMoldError):
+            sim.write_trail(t)
+        with pytest.raises(slm.SlimeMoldError):
+            sim.write_trail(np.zeros((10, 10), np.float32))
+        # -0 is accepted and stored as +0 (the sign bit is the deposit mark)
+        t = np.zeros((64, 64), np.float32)
+        t[2, 2] = -0.0
+        sim.write_trail(t)
+        assert not np.signbit(sim.read_trail()).any()
+        assert np.array_equal(sim.read_agents(), a)

@@ -1,0 +1,135 @@
+"""Pins for the oracle's primitives, each against something independent of it.
+
+sin/cos: libm's double sin/cos and mpmath (error bound, symmetry, specials).
+Storage rounding: numpy's float16 cast and torch's bfloat16 cast (both RNE).
+SplitMix64: the generator's published first outputs for seed 0.
+WGSL '%', mix, fract, sign, i32(): special cases whose answers the definitions fix.
+"""
+import math
+import struct
+
+import numpy as np
+import pytest
+
+
+def f32(x):
+    return struct.unpack("<f", struct.pack("<f", x))[0]
+
+
+def ulp(x):
+    x = abs(f32(x))
+    return float(np.spacing(np.float32(x))) if x > 0 else 1.4e-45
+
+
+def test_sincos_accuracy_dense_small_range(orc):
+    # Dense sample of the range the sensors and moves use; the exhaustive run over
+    # every float measured max 1.43 ulp on [0, 8) and 1.80 ulp overall.
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([np.linspace(-8.0, 8.0, 20001, dtype=np.float32),
+                         rng.uniform(-20, 20, 20000).astype(np.float32)])
+    worst = 0.0
+    for x in xs:
+        s, c = orc.sincos(float(x))
+        rs, rc = math.sin(float(x)), math.cos(float(x))
+        worst = max(worst, abs(s - rs) / ulp(rs), abs(c - rc) / ulp(rc))
+    assert worst <= 1.5, worst
+
+
+def test_sincos_large_arguments_vs_mpmath(orc):
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 60
+    rng = np.random.default_rng(2)
+    xs = [4096.0, 4096.5, 1e5, 3.4906585e9, 2.0 ** 31, 2.0 ** 32 - 256, 1e20, 3.4e38]
+    xs += list(rng.uniform(4096, 4.3e9, 300).astype(np.float32))
+    for x in xs:
+        x = f32(float(x))
+        s, c = orc.sincos(x)
+        rs, rc = float(mp.sin(mp.mpf(x))), float(mp.cos(mp.mpf(x)))
+        assert abs(s - rs) <= 2.0 * ulp(rs), (x, s, rs)
+        assert abs(c - rc) <= 2.0 * ulp(rc), (x, c, rc)
+
+
+def test_sincos_symmetry_and_specials(orc):
+    for x in [0.1, 1.0, 2.5, 7.9, 1234.5, 5e6, 3e9]:
+        s, c = orc.sincos(x)
+        ns, nc = orc.sincos(-x)
+        assert ns == -s and nc == c
+    s, c = orc.sincos(0.0)
+    assert s == 0.0 and c == 1.0
+    s, _ = orc.sincos(-0.0)
+    assert math.copysign(1.0, s) == -1.0
+    for x in (math.inf, -math.inf, math.nan):
+        s, c = orc.sincos(x)
+        assert math.isnan(s) and math.isnan(c)
+    # quadrant boundaries: sin(pi/2) = cos(0) = 1, cos(pi) = -1
+    assert orc.sincos(f32(math.pi / 2))[0] == 1.0
+    assert orc.sincos(f32(math.pi))[1] == -1.0
+
+
+def test_wgsl_mod_special_cases(orc):
+    L = orc.lib()
+    tau = f32(6.28318530718)
+    assert L.orc_wgsl_mod(1.0, tau) == 1.0
+    assert L.orc_wgsl_mod(tau + 1.0, tau) == f32(f32(tau + 1.0) - tau)  # exact by Sterbenz
+    assert L.orc_wgsl_mod(-1.0, tau) == -1.0                           # truncated, keeps sign
+    assert L.orc_wgsl_mod(7.5, 2.0) == 1.5
+    assert L.orc_wgsl_mod(-7.5, 2.0) == -1.5
+    # just below b: a/b stays below 1 (ulp(a)/b > 2^-25 for every float a < b),
+    # so trunc gives 0 and the result is a itself
+    for b in (tau, 1024.0, 1600.0, 900.0, 3.0):
+        below = float(np.nextafter(np.float32(b), np.float32(0)))
+        assert L.orc_wgsl_mod(below, b) == below
+    # wrap of a position exactly at the width gives 0
+    assert L.orc_wgsl_mod(1024.0, 1024.0) == 0.0
+
+
+def test_mix_fract_sign_i32(orc):
+    L = orc.lib()
+    assert L.orc_mix(2.0, 6.0, 0.25) == 3.0
+    assert L.orc_mix(0.7, 0.2, 0.0) == f32(0.7)
+    assert L.orc_mix(0.7, 0.2, 1.0) == f32(0.2)
+    assert L.orc_fract(2.75) == 0.75
+    assert L.orc_fract(-0.25) == 0.75
+    assert L.orc_sign(3.0) == 1.0 and L.orc_sign(-2.0) == -1.0 and L.orc_sign(0.0) == 0.0
+    assert L.orc_f32_to_i32(3.9) == 3 and L.orc_f32_to_i32(-3.9) == -3
+    assert L.orc_f32_to_i32(3e10) == 2 ** 31 - 1 and L.orc_f32_to_i32(-3e10) == -2 ** 31
+    assert L.orc_f32_to_i32(math.nan) == 0
+
+
+def test_f16_rounding_matches_numpy(orc):
+    L = orc.lib()
+    rng = np.random.default_rng(3)
+    vals = np.concatenate([rng.uniform(0, 1, 20000), rng.uniform(0, 1e-4, 5000),
+                           rng.uniform(0, 7e4, 2000), [0.0, 6.1e-5, 5.96e-8, 2.98e-8, 65504.0,
+                                                       65519.99, 65520.0, 1.0, 0.5]])
+    vals = vals.astype(np.float32)
+    for v in vals:
+        want = np.float16(v).view(np.uint16)
+        assert L.orc_f32_to_f16(float(v)) == int(want), v
+        assert L.orc_f16_to_f32(int(want)) == float(np.float16(v).astype(np.float32))
+
+
+def test_bf16_rounding_matches_torch(orc):
+    torch = pytest.importorskip("torch")
+    L = orc.lib()
+    rng = np.random.default_rng(4)
+    vals = np.concatenate([rng.uniform(0, 1, 20000), rng.uniform(0, 1e-30, 2000),
+                           [0.0, 1.0, 0.00390625, 1.00390625, 1.01171875, 3e38]]).astype(np.float32)
+    t = torch.from_numpy(vals).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    for v, want in zip(vals, t):
+        assert L.orc_f32_to_bf16(float(v)) == int(want), v
+        assert L.orc_bf16_to_f32(int(want)) == float(np.float32(struct.unpack("<f", struct.pack("<I", int(want) << 16))[0]))
+
+
+def test_splitmix64_published_outputs(orc):
+    L = orc.lib()
+    assert [L.orc_splitmix64_at(0, n) for n in range(3)] == [
+        0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_uniform24_resolution_and_range(orc):
+    L = orc.lib()
+    us = np.array([L.orc_uniform24(42, n) for n in range(20000)])
+    assert us.min() >= 0.0 and us.max() < 1.0
+    assert np.all((us * 2 ** 24) == np.floor(us * 2 ** 24))  # multiples of 2^-24
+    assert abs(us.mean() - 0.5) < 0.01
