@@ -48,8 +48,9 @@ def parse():
     ap.add_argument("--agents-log2", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
-    ap.add_argument("--resort", type=int, default=0, help="Morton re-sort interval (0: off)")
+    ap.add_argument("--resort", type=int, default=20, help="Morton re-sort interval (0: off)")
     ap.add_argument("--graph-chunk", type=int, default=100)
+    ap.add_argument("--sort-cell-log2", type=int, default=4, help="re-sort bin edge 2^k cells")
     ap.add_argument("--timing-steps", type=int, default=30,
                     help="eager steps per preset for the per-kernel event timing")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -92,7 +93,8 @@ def main():
     seg = [K // 3, K // 3, K - 2 * (K // 3)]
 
     sim = slimemold.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
-                               resort_interval=args.resort, graph_chunk=args.graph_chunk)
+                               resort_interval=args.resort, graph_chunk=args.graph_chunk,
+                               sort_cell_log2=args.sort_cell_log2)
     print(f"[bench] {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
           f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}", file=sys.stderr,
           flush=True)
@@ -164,6 +166,7 @@ def main():
                          f"seed {args.seed}, hipGraph {args.graph_chunk}-step chunks"),
             "width": W, "height": H, "agents": N, "storage": args.storage, "seed": args.seed,
             "presets": PRESETS, "graph_chunk": args.graph_chunk, "resort_interval": args.resort,
+            "sort_cell_log2": args.sort_cell_log2,
         },
         "roofline": roof,
         "diffuse": {

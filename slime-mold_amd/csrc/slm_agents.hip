@@ -4,10 +4,14 @@
 //
 // One lane per agent over structure-of-arrays state.  Sensing reads the trail
 // as it was when the step began: a deposit does not write a value, it sets the
-// sign bit of the target cell with a non-returning atomic OR (trail values are
-// never negative, so the bit is free); sensing reads |T|, and the trail pass
-// that follows turns marked cells into clamp(T + deposit, 0, 1).  The result is
-// therefore independent of the order agents run in.
+// sign bit of the target cell (trail values are never negative, so the bit is
+// free); sensing reads |T|, and the trail pass that follows turns marked cells
+// into clamp(|T| + deposit, 0, 1).  The mark is a plain load and store of that
+// one cell, not an atomic: every agent landing in a cell stores the same bits
+// (-|T|), and a sensor reading the cell meanwhile sees T or -T, whose |.| is the
+// same, so the race is benign.  The result is therefore independent of the
+// order agents run in.  (A global atomic OR per agent cost ~0.45 ms per 2^24
+// agents and serialised badly once agents are spatially sorted.)
 //
 // Arithmetic follows the reference shader operation by operation: this file is
 // compiled with -ffp-contract=off, division is IEEE, '%' is a - b*trunc(a/b),
@@ -26,38 +30,52 @@ constexpr int kAgentBlock = 256;
 template <int S>
 struct TrailRead;
 
+// load_pair(t, i) returns |T[i]| and |T[i+1]| from one (4- or 2-byte aligned)
+// load; gfx950 allows unaligned global loads, and the compiler only fuses the
+// two cells into one instruction where the target permits it.
 template <>
 struct TrailRead<SLM_STORAGE_F32> {
-  __device__ static float load(const void* t, uint32_t i) {
-    uint32_t b = static_cast<const uint32_t*>(t)[i];
-    return __uint_as_float(b & 0x7fffffffu);
+  __device__ static void load_pair(const void* t, uint32_t i, float& a, float& b) {
+    uint2 v;
+    __builtin_memcpy(&v, static_cast<const uint32_t*>(t) + i, sizeof v);
+    a = __uint_as_float(v.x & 0x7fffffffu);
+    b = __uint_as_float(v.y & 0x7fffffffu);
   }
   __device__ static void mark(void* t, uint32_t i) {
-    __hip_atomic_fetch_or(static_cast<uint32_t*>(t) + i, 0x80000000u, __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t* p = static_cast<uint32_t*>(t) + i;
+    *p = *p | 0x80000000u;
   }
 };
 
+// 16-bit cells are marked with a 2-byte store, which leaves the neighbour in
+// the same 32-bit word untouched.
 __device__ inline void mark16(void* t, uint32_t i) {
-  uint32_t bit = (i & 1u) ? 0x80000000u : 0x00008000u;
-  __hip_atomic_fetch_or(static_cast<uint32_t*>(t) + (i >> 1), bit, __ATOMIC_RELAXED,
-                        __HIP_MEMORY_SCOPE_AGENT);
+  uint16_t* p = static_cast<uint16_t*>(t) + i;
+  *p = static_cast<uint16_t>(*p | 0x8000u);
+}
+
+__device__ inline uint32_t load_u16_pair(const void* t, uint32_t i) {
+  uint32_t w;
+  __builtin_memcpy(&w, static_cast<const uint16_t*>(t) + i, sizeof w);
+  return w;
 }
 
 template <>
 struct TrailRead<SLM_STORAGE_BF16> {
-  __device__ static float load(const void* t, uint32_t i) {
-    uint32_t h = static_cast<const uint16_t*>(t)[i];
-    return __uint_as_float((h & 0x7fffu) << 16);
+  __device__ static void load_pair(const void* t, uint32_t i, float& a, float& b) {
+    uint32_t w = load_u16_pair(t, i);
+    a = __uint_as_float((w & 0x7fffu) << 16);
+    b = __uint_as_float(((w >> 16) & 0x7fffu) << 16);
   }
   __device__ static void mark(void* t, uint32_t i) { mark16(t, i); }
 };
 
 template <>
 struct TrailRead<SLM_STORAGE_F16> {
-  __device__ static float load(const void* t, uint32_t i) {
-    uint16_t h = static_cast<const uint16_t*>(t)[i] & 0x7fffu;
-    return static_cast<float>(__builtin_bit_cast(_Float16, h));
+  __device__ static void load_pair(const void* t, uint32_t i, float& a, float& b) {
+    uint32_t w = load_u16_pair(t, i);
+    a = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0x7fffu)));
+    b = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>((w >> 16) & 0x7fffu)));
   }
   __device__ static void mark(void* t, uint32_t i) { mark16(t, i); }
 };
@@ -84,23 +102,27 @@ __device__ inline float wgsl_clamp(float e, float lo, float hi) {
 
 // Bilinear trail sample, not toroidal (compute.wgsl:7-29).  Positions are kept
 // finite and within [-2^24, 2^24] by the API, so the i32 conversions are exact.
+// Branch-free: an out-of-range sample loads cells 0 and 1 twice (the trail
+// allocation is at least 256 bytes) and returns 0, so the six tap-pair loads of
+// an agent's three sensors are all in flight at once instead of one sample's
+// loads waiting behind the previous sample's branch.
 template <int S>
 __device__ inline float sample(const void* t, uint32_t w, uint32_t h, float px, float py) {
   int x0 = static_cast<int>(__builtin_floorf(px));
   int y0 = static_cast<int>(__builtin_floorf(py));
-  if (x0 < 0 || static_cast<uint32_t>(x0) >= w - 1u || y0 < 0 ||
-      static_cast<uint32_t>(y0) >= h - 1u)
-    return 0.0f;
+  bool in = x0 >= 0 && static_cast<uint32_t>(x0) < w - 1u && y0 >= 0 &&
+            static_cast<uint32_t>(y0) < h - 1u;
   float dx = px - static_cast<float>(x0);
   float dy = py - static_cast<float>(y0);
-  uint32_t i00 = static_cast<uint32_t>(y0) * w + static_cast<uint32_t>(x0);
-  float v00 = TrailRead<S>::load(t, i00);
-  float v10 = TrailRead<S>::load(t, i00 + 1u);
-  float v01 = TrailRead<S>::load(t, i00 + w);
-  float v11 = TrailRead<S>::load(t, i00 + w + 1u);
+  uint32_t i00 = in ? static_cast<uint32_t>(y0) * w + static_cast<uint32_t>(x0) : 0u;
+  uint32_t down = in ? w : 0u;
+  float v00, v10, v01, v11;
+  TrailRead<S>::load_pair(t, i00, v00, v10);
+  TrailRead<S>::load_pair(t, i00 + down, v01, v11);
   float v0 = wgsl_mix(v00, v10, dx);
   float v1 = wgsl_mix(v01, v11, dx);
-  return wgsl_mix(v0, v1, dy);
+  float v = wgsl_mix(v0, v1, dy);
+  return in ? v : 0.0f;
 }
 
 // Workgroups of one launch are dealt round-robin to the 8 XCDs; give XCD k a
@@ -128,7 +150,6 @@ __global__ __launch_bounds__(kAgentBlock) void agents_kernel(AgentArgs args) {
   const uint32_t w = args.w, h = args.h;
 
   float x = args.a.x[j], y = args.a.y[j], angle = args.a.th[j], speed = args.a.v[j];
-  uint32_t index = args.a.id ? args.a.id[j] : j;
 
   float sp = wgsl_clamp(speed, p.speed_min, p.speed_max);  // :72
   if (sp != speed) args.a.v[j] = sp;
@@ -161,6 +182,7 @@ __global__ __launch_bounds__(kAgentBlock) void agents_kernel(AgentArgs args) {
   // :115-118.  With zero jitter the term is (finite) * 0 = +-0, which leaves
   // the angle unchanged, so the hash is skipped.
   if (p.jitter != 0.0f) {
+    uint32_t index = args.a.id ? args.a.id[j] : j;  // original index (S6)
     float hx = static_cast<float>(static_cast<int32_t>(index)) * 12.9898f;
     hx = hx + x * 78.233f;
     hx = hx + y * 37.719f;

@@ -44,13 +44,32 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
+// Bins each agent and takes its rank inside the bin.  Lanes of a wave that share
+// a bin are served by one atomic (agents re-sorted earlier arrive grouped by
+// bin, so a wave usually holds one to four bins); ranks follow lane order.
 __global__ void count_kernel(AgentSoA a, uint32_t n, BinGeom b, uint32_t* key, uint32_t* rank,
                              uint32_t* count) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  uint32_t k = bin_of(a.x[j], a.y[j], b);
-  key[j] = k;
-  rank[j] = atomicAdd(count + k, 1u);
+  bool active = j < n;
+  uint32_t k = active ? bin_of(a.x[j], a.y[j], b) : 0u;
+  uint32_t lane = threadIdx.x & 63u;
+  uint64_t below = (uint64_t(1) << lane) - 1u;
+  uint64_t pending = __ballot(active);
+  uint32_t r = 0;
+  while (pending) {
+    int leader = __builtin_ctzll(pending);
+    uint32_t lk = __shfl(k, leader);
+    uint64_t same = __ballot(active && k == lk);
+    uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(count + lk, __popcll(same));
+    base = __shfl(base, leader);
+    if (active && k == lk) r = base + __popcll(same & below);
+    pending &= ~same;
+  }
+  if (active) {
+    key[j] = k;
+    rank[j] = r;
+  }
 }
 
 // Exclusive scan of one tile of counts; writes the tile total to sums[tile].
