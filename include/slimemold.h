@@ -95,6 +95,11 @@ typedef struct slm_config {
 
 /* Map consecutive workgroups of a launch onto one XCD's L2 (default on). */
 #define SLM_FLAG_NO_XCD_REMAP 0x1u
+/* Keep agent positions in f64 (for grids of 8192 and larger, where an f32
+ * position's ulp approaches 1e-3 cell).  Sensor offsets and the move step stay
+ * the shader's f32 products; they are added to the f64 position, and the wrap,
+ * floor and deposit cell are taken in f64.  Agents are still exchanged as f32. */
+#define SLM_FLAG_F64_POSITIONS 0x2u
 
 typedef struct slm_info {
   uint32_t width, height;
@@ -188,6 +193,60 @@ int slm_write_trail(slm_sim* sim, const float* trail, uint64_t cells);
 int slm_total_mass(slm_sim* sim, double* out);
 
 int slm_get_info(const slm_sim* sim, slm_info* out);
+
+/* ---- batches ---------------------------------------------------------------
+ * Many independent simulations of uneven sizes advanced together: every step
+ * is one agent launch and one trail launch over all instances, driven by an
+ * instance-offset table (no per-instance launches).  Instance k of a batch
+ * gives bit for bit the result of a single simulation created with the same
+ * width, height, agent count, seed and parameters.  No counterpart in the
+ * reference, which runs one simulation per process (src/main.rs:147). */
+typedef struct slm_instance_desc {
+  uint32_t width, height; /* as slm_config */
+  uint64_t agent_count;
+  uint64_t seed;
+  slm_params params;
+} slm_instance_desc;
+
+typedef struct slm_batch_config {
+  int32_t storage;          /* slm_storage, shared by all instances */
+  int32_t device;
+  uint32_t resort_interval; /* as slm_config */
+  uint32_t graph_chunk;
+  uint32_t sort_cell_log2;
+  uint32_t flags;           /* SLM_FLAG_* */
+  uint32_t reserved[6];     /* must be zero */
+} slm_batch_config;
+
+typedef struct slm_batch_info {
+  uint32_t count;           /* instances */
+  int32_t storage;
+  uint64_t total_agents;
+  uint64_t total_cells;
+  uint64_t steps_done;
+  uint64_t device_bytes;
+} slm_batch_info;
+
+typedef struct slm_batch slm_batch;
+
+int slm_batch_config_default(slm_batch_config* cfg);
+/* 1 <= count <= 65535 instances; total agents < 2^32, total cells <= 2^40. */
+int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* instances,
+                     uint32_t count, slm_batch** out);
+int slm_batch_destroy(slm_batch* batch);
+int slm_batch_step(slm_batch* batch, uint32_t n_steps);
+int slm_batch_step_timed(slm_batch* batch, uint32_t n_steps, slm_timing* out);
+int slm_batch_synchronize(slm_batch* batch);
+int slm_batch_set_params(slm_batch* batch, uint32_t index, const slm_params* params);
+int slm_batch_clear(slm_batch* batch, uint32_t index);
+int slm_batch_read_agents(slm_batch* batch, uint32_t index, float* agents_xyzw, uint64_t n);
+int slm_batch_write_agents(slm_batch* batch, uint32_t index, const float* agents_xyzw,
+                           uint64_t n);
+int slm_batch_read_trail(slm_batch* batch, uint32_t index, float* trail, uint64_t cells);
+int slm_batch_write_trail(slm_batch* batch, uint32_t index, const float* trail, uint64_t cells);
+/* f64 trail sums of all instances: out[k] for instance k, count == instances. */
+int slm_batch_total_mass(slm_batch* batch, double* out, uint32_t count);
+int slm_batch_get_info(const slm_batch* batch, slm_batch_info* out);
 
 #ifdef __cplusplus
 }

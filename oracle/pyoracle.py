@@ -61,6 +61,9 @@ def lib():
             "orc_step_f64": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _D,
                                             _D, ctypes.POINTER(OrcParams), ctypes.c_uint32,
                                             ctypes.c_int]),
+            "orc_step_p64": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _D,
+                                            _F, ctypes.POINTER(OrcParams), ctypes.c_int,
+                                            ctypes.c_uint32, ctypes.c_int]),
             "orc_total_mass": (ctypes.c_double, [_F, ctypes.c_uint64]),
             "orc_total_mass_f64": (ctypes.c_double, [_D, ctypes.c_uint64]),
         }
@@ -143,6 +146,19 @@ def step(agents: np.ndarray, trail: np.ndarray, p, n_steps: int = 1, storage: st
                         ctypes.byref(op), STORAGE[storage], n_steps, nthreads)
     if rc != 0:
         raise RuntimeError(f"orc_step failed: {rc}")
+
+
+def step_p64(agents: np.ndarray, trail: np.ndarray, p, n_steps: int = 1, storage: str = "f32",
+             nthreads: int = 0) -> None:
+    """f64-position variant: (n,4) float64 agents, (h,w) float32 trail, in place."""
+    assert agents.dtype == np.float64 and agents.flags.c_contiguous
+    assert trail.dtype == np.float32 and trail.flags.c_contiguous
+    op = params(p)
+    rc = lib().orc_step_p64(trail.shape[1], trail.shape[0], agents.shape[0],
+                            agents.ctypes.data_as(_D), _f(trail), ctypes.byref(op),
+                            STORAGE[storage], n_steps, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"orc_step_p64 failed: {rc}")
 
 
 def step_f64(agents: np.ndarray, trail: np.ndarray, p, n_steps: int = 1, nthreads: int = 0) -> None:

@@ -240,6 +240,80 @@ int64_t orc_agent_update(const float* t, uint32_t w, uint32_t h, const orc_param
   return dep;
 }
 
+/* f64-position variant (the library's SLM_FLAG_F64_POSITIONS): x, y in double;
+ * sensor offsets and the move step are the f32 products of the f32 path, added
+ * to the double position; floor, wrap and deposit cell in double; bilinear
+ * weights and the jitter hash use the f32 roundings.  agent4 = (x, y, angle,
+ * speed) with angle and speed holding f32 values. */
+static float sample_p64(const float* t, uint32_t w, uint32_t h, double px, double py) {
+  double fx = floor(px), fy = floor(py);
+  float dx = (float)(px - fx), dy = (float)(py - fy);
+  int32_t x0 = (fx != fx) ? 0 : (fx >= 2147483648.0 ? 2147483647 : (fx <= -2147483648.0 ? (int32_t)0x80000000u : (int32_t)fx));
+  int32_t y0 = (fy != fy) ? 0 : (fy >= 2147483648.0 ? 2147483647 : (fy <= -2147483648.0 ? (int32_t)0x80000000u : (int32_t)fy));
+  int64_t x1 = (int64_t)x0 + 1, y1 = (int64_t)y0 + 1;
+  if (x0 < 0 || x1 >= (int64_t)w || y0 < 0 || y1 >= (int64_t)h) return 0.0f;
+  float v00 = t[(int64_t)y0 * w + x0], v10 = t[(int64_t)y0 * w + x1];
+  float v01 = t[y1 * w + x0], v11 = t[y1 * w + x1];
+  return orc_mix(orc_mix(v00, v10, dx), orc_mix(v01, v11, dx), dy);
+}
+
+static double mod_d(double a, double b) {
+  double q = a / b;
+  double tr = trunc(q);
+  double pr = b * tr;
+  return a - pr;
+}
+
+int64_t orc_agent_update_p64(const float* t, uint32_t w, uint32_t h, const orc_params* p,
+                             uint32_t index, double* ag) {
+  double xd = ag[0], yd = ag[1];
+  float angle = (float)ag[2], speed = (float)ag[3];
+  float x = (float)xd, y = (float)yd;
+  speed = wgsl_clamp(speed, p->agent_speed_min, p->agent_speed_max);
+  float sa = p->agent_sensor_angle, sd = p->agent_sensor_distance;
+  float sl, cl, sr, cr, sc, cc;
+  slm_sincosf(angle - sa, &sl, &cl);
+  slm_sincosf(angle + sa, &sr, &cr);
+  slm_sincosf(angle, &sc, &cc);
+  float lv = sample_p64(t, w, h, xd + (double)(sd * cl), yd + (double)(sd * sl));
+  float rv = sample_p64(t, w, h, xd + (double)(sd * cr), yd + (double)(sd * sr));
+  float cv = sample_p64(t, w, h, xd + (double)(sd * cc), yd + (double)(sd * sc));
+  if (cv > lv && cv > rv) {
+  } else if (lv > rv) {
+    float target = angle - ORC_TAU;
+    float diff = target - angle;
+    angle += wgsl_min(p->agent_turn_speed, fabsf(diff)) * orc_sign(diff);
+  } else if (rv > lv) {
+    float target = angle + ORC_TAU;
+    float diff = target - angle;
+    angle += wgsl_min(p->agent_turn_speed, fabsf(diff)) * orc_sign(diff);
+  }
+  float hx = (float)(int32_t)index * 12.9898f;
+  hx = hx + x * 78.233f;
+  hx = hx + y * 37.719f;
+  float rnd = orc_fract(slm_sinf(hx) * 43758.5453f);
+  angle += (rnd * 2.0f - 1.0f) * p->agent_jitter;
+  angle = orc_wgsl_mod(angle, ORC_TAU);
+  if (angle < 0.0f) angle = angle + ORC_TAU;
+  float move = speed * ORC_TIME_STEP;
+  float ms, mc;
+  slm_sincosf(angle, &ms, &mc);
+  double fw = (double)w, fh = (double)h;
+  xd = xd + (double)(move * mc);
+  yd = yd + (double)(move * ms);
+  xd = mod_d(xd, fw);
+  if (xd < 0.0) xd = xd + fw;
+  yd = mod_d(yd, fh);
+  if (yd < 0.0) yd = yd + fh;
+  int64_t dep = -1;
+  if (xd >= 0.0 && xd < fw && yd >= 0.0 && yd < fh) dep = (int64_t)yd * w + (int64_t)xd;
+  ag[0] = xd;
+  ag[1] = yd;
+  ag[2] = angle;
+  ag[3] = speed;
+  return dep;
+}
+
 /* ---- trail pass ---------------------------------------------------------- */
 
 void orc_trail_pass(const float* tin, const uint8_t* mark, uint32_t w, uint32_t h,
@@ -294,6 +368,36 @@ int orc_step(uint32_t w, uint32_t h, uint64_t n, float* ag, float* trail, const 
 #pragma omp parallel for schedule(static) num_threads(nt)
     for (int64_t i = 0; i < (int64_t)n; i++) {
       int64_t d = orc_agent_update(trail, w, h, p, (uint32_t)i, ag + 4 * i);
+      if (d >= 0) {
+#pragma omp atomic write
+        mark[d] = 1;
+      }
+    }
+    orc_trail_pass(trail, mark, w, h, p, storage, next, nt);
+    memcpy(trail, next, cells * sizeof(float));
+  }
+  free(mark);
+  free(next);
+  return 0;
+}
+
+int orc_step_p64(uint32_t w, uint32_t h, uint64_t n, double* ag, float* trail,
+                 const orc_params* p, int storage, uint32_t n_steps, int nthreads) {
+  uint64_t cells = (uint64_t)w * h;
+  if (w == 0 || h == 0) return -1;
+  uint8_t* mark = (uint8_t*)malloc(cells);
+  float* next = (float*)malloc(cells * sizeof(float));
+  if (!mark || !next) {
+    free(mark);
+    free(next);
+    return -2;
+  }
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+  for (uint32_t s = 0; s < n_steps; s++) {
+    memset(mark, 0, cells);
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+      int64_t d = orc_agent_update_p64(trail, w, h, p, (uint32_t)i, ag + 4 * i);
       if (d >= 0) {
 #pragma omp atomic write
         mark[d] = 1;

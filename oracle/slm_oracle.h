@@ -91,6 +91,13 @@ void orc_trail_pass(const float* trail_in, const uint8_t* mark, uint32_t w, uint
 int orc_step(uint32_t w, uint32_t h, uint64_t n, float* agents_aos, float* trail,
              const orc_params* p, int storage, uint32_t n_steps, int nthreads);
 
+/* f64-position variant of orc_agent_update / orc_step: agents as (x, y,
+ * angle, speed) doubles, angle and speed holding f32 values. */
+int64_t orc_agent_update_p64(const float* trail, uint32_t w, uint32_t h, const orc_params* p,
+                             uint32_t index, double* agent4);
+int orc_step_p64(uint32_t w, uint32_t h, uint64_t n, double* agents_aos, float* trail,
+                 const orc_params* p, int storage, uint32_t n_steps, int nthreads);
+
 /* The same step in double precision (agents as (x,y,angle,speed) doubles). */
 int orc_step_f64(uint32_t w, uint32_t h, uint64_t n, double* agents_aos, double* trail,
                  const orc_params* p, uint32_t n_steps, int nthreads);

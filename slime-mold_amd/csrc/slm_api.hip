@@ -1,14 +1,19 @@
 // slm_api.hip - the C ABI declared in include/slimemold.h.
 //
-// A handle owns: the parameter block (device), two trail buffers used
-// alternately (the trail pass reads one and writes the other), the agents as
-// SoA arrays (two sets when re-sorting, which scatters from one to the other),
-// re-sort scratch, f64 mass scratch and one non-blocking HIP stream.  Steps are
-// enqueued eagerly or as captured hipGraphs of `graph_chunk` steps; a graph is
-// keyed by which trail and agent buffers are current when it starts, so at
-// most four exist per handle.  Every entry point validates its arguments before
-// any kernel is launched, so the kernels can rely on in-range shapes and
-// finite, bounded positions.
+// Both handle types wrap one Engine: a pool of instances (a single simulation
+// is a pool of one) laid out in shared device buffers -
+//   trail[2]      every instance's trail, used alternately (the trail pass
+//                 reads one and writes the other);
+//   agents[2]     SoA agent arrays, instance by instance (the second set only
+//                 when re-sorting, which scatters from one set to the other);
+//   tables        the instance table, per-instance parameters and the
+//                 block -> first-instance maps the kernels walk;
+// plus re-sort scratch, f64 mass scratch and one non-blocking HIP stream.
+// Steps are enqueued eagerly or as captured hipGraphs of `graph_chunk` steps;
+// a graph is keyed by which trail and agent buffers are current when it
+// starts, so at most four exist per handle.  Every entry point validates its
+// arguments before any kernel is launched, so the kernels can rely on
+// in-range shapes and finite, bounded positions.
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -35,19 +40,28 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define SLM_HIP(call)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(SLM_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
-                  __FILE__, __LINE__);                                                  \
+#define SLM_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(SLM_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                \
   } while (0)
 
-constexpr uint32_t kMaxSide = 1u << 24;            // f32(width) exact
-constexpr uint64_t kMaxCells = 1ull << 31;          // i32 cell index (compute.wgsl:21)
-constexpr uint64_t kMaxAgents = 0xffffff00ull;      // u32 slot index, whole blocks
-constexpr float kMaxSensorDistance = 4194304.0f;    // keeps sensor taps inside i32
-constexpr uint32_t kMaxSortBins = 1u << 20;
+#define SLM_TRY(expr)         \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != SLM_OK) return rc_; \
+  } while (0)
+
+constexpr uint32_t kMaxSide = 1u << 24;          // f32(width) exact
+constexpr uint64_t kMaxCells = 1ull << 31;        // i32 cell index (compute.wgsl:21)
+constexpr uint64_t kMaxPoolCells = 1ull << 40;
+constexpr uint64_t kMaxAgents = 0xffffff00ull;    // u32 slot index, whole blocks
+constexpr float kMaxSensorDistance = 4194304.0f;  // keeps sensor taps inside i32
+constexpr uint64_t kMaxTotalBins = 1ull << 30;
+constexpr uint32_t kTargetWaves = 8192;           // trail-pass units per pool
+constexpr uint32_t kKnownFlags = SLM_FLAG_NO_XCD_REMAP | SLM_FLAG_F64_POSITIONS;
 
 bool finite_params(const slm_params& p) {
   const float* f = &p.decay_factor;
@@ -73,33 +87,6 @@ int check_grid(uint32_t w, uint32_t h) {
   return SLM_OK;
 }
 
-struct Graph {
-  hipGraphExec_t exec = nullptr;
-};
-
-}  // namespace
-
-struct slm_sim {
-  slm_config cfg{};
-  slm_params params{};
-  hipStream_t stream = nullptr;
-  DevParams* d_params = nullptr;
-  void* trail[2] = {nullptr, nullptr};
-  int cur = 0;                 // trail[cur] holds the current trail
-  AgentSoA agents[2] = {};     // agents[acur] is current; [1] only when re-sorting
-  int acur = 0;
-  SortScratch sort{};
-  double* mass_partials = nullptr;
-  double* mass_out = nullptr;
-  uint64_t steps_done = 0;
-  uint64_t reassign_calls = 0;
-  uint64_t device_bytes = 0;
-  Graph graphs[2][2];          // [trail parity][agent parity] at chunk start
-  std::vector<void*> allocations;
-};
-
-namespace {
-
 struct DeviceGuard {
   int prev = -1;
   explicit DeviceGuard(int dev) {
@@ -111,202 +98,624 @@ struct DeviceGuard {
   }
 };
 
-int dev_alloc(slm_sim* s, void** p, size_t bytes) {
-  *p = nullptr;
-  if (bytes == 0) bytes = 256;
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    (void)hipGetLastError();
-    return fail(e == hipErrorOutOfMemory ? SLM_ERR_OUT_OF_MEMORY : SLM_ERR_HIP,
-                "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-  }
-  s->allocations.push_back(*p);
-  s->device_bytes += bytes;
+// Settings shared by every instance of an engine.
+struct EngineConfig {
+  int storage = SLM_STORAGE_F32;
+  int device = 0;
+  uint32_t resort_interval = 0;
+  uint32_t graph_chunk = 0;
+  uint32_t sort_cell_log2 = 4;
+  uint32_t flags = 0;
+};
+
+int check_engine_config(const EngineConfig& c, const uint32_t* reserved, size_t nreserved) {
+  if (c.storage < SLM_STORAGE_F32 || c.storage > SLM_STORAGE_F16)
+    return fail(SLM_ERR_INVALID_ARGUMENT, "unknown storage %d", c.storage);
+  if (c.sort_cell_log2 > 12) return fail(SLM_ERR_INVALID_ARGUMENT, "sort_cell_log2 must be <= 12");
+  if (c.graph_chunk > 100000) return fail(SLM_ERR_INVALID_ARGUMENT, "graph_chunk must be <= 100000");
+  if (c.flags & ~kKnownFlags) return fail(SLM_ERR_INVALID_ARGUMENT, "unknown flags");
+  for (size_t i = 0; i < nreserved; i++)
+    if (reserved[i]) return fail(SLM_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
   return SLM_OK;
 }
 
-void dev_free(slm_sim* s, void* p, size_t bytes) {
-  if (!p) return;
-  for (auto& a : s->allocations)
-    if (a == p) a = nullptr;
-  (void)hipFree(p);
-  s->device_bytes -= bytes ? bytes : 256;
-}
+struct InstanceSpec {
+  uint32_t width, height;
+  uint64_t agents;
+  uint64_t seed;
+  slm_params params;
+};
 
-uint64_t cells_of(const slm_sim* s) { return static_cast<uint64_t>(s->cfg.width) * s->cfg.height; }
-// Rounded up to 256 bytes: the 16-bit deposit mark is a 32-bit atomic on the
-// word holding the cell, which for the last cell of an odd-sized grid reaches
-// two bytes past width*height*2.
-uint64_t trail_bytes(const slm_sim* s) {
-  return (cells_of(s) * storage_bytes(s->cfg.storage) + 255u) & ~uint64_t(255u);
-}
-bool sorting(const slm_sim* s) { return s->cfg.resort_interval > 0; }
+class Engine {
+ public:
+  EngineConfig cfg;
+  std::vector<InstanceSpec> spec;
+  std::vector<Instance> inst;      // host copy of the device table
+  std::vector<uint64_t> reassigns; // slm_reassign_speeds calls per instance
+  hipStream_t stream = nullptr;
+  uint64_t steps_done = 0;
+  uint64_t device_bytes = 0;
 
-StepGeom geom(const slm_sim* s) {
-  StepGeom g;
-  g.width = s->cfg.width;
-  g.height = s->cfg.height;
-  g.agents = static_cast<uint32_t>(s->cfg.agent_count);
-  g.storage = s->cfg.storage;
-  g.xcd_remap = (s->cfg.flags & SLM_FLAG_NO_XCD_REMAP) ? 0 : 1;
-  return g;
-}
+  ~Engine() { release(); }
 
-void drop_graphs(slm_sim* s) {
-  for (auto& row : s->graphs)
-    for (auto& g : row)
-      if (g.exec) {
-        (void)hipGraphExecDestroy(g.exec);
-        g.exec = nullptr;
+  int create(const EngineConfig& c, std::vector<InstanceSpec> s) {
+    cfg = c;
+    spec = std::move(s);
+    reassigns.assign(spec.size(), 0);
+    SLM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    SLM_TRY(alloc_const());
+    SLM_TRY(layout_agents());
+    SLM_TRY(layout_trail());
+    SLM_TRY(upload_params_all());
+    for (size_t k = 0; k < spec.size(); k++) init_agents(k);
+    return sync_checked("initialisation");
+  }
+
+  bool pos64() const { return (cfg.flags & SLM_FLAG_F64_POSITIONS) != 0; }
+  bool sorting() const { return cfg.resort_interval > 0; }
+  uint32_t count() const { return static_cast<uint32_t>(spec.size()); }
+  uint64_t cells(size_t k) const { return static_cast<uint64_t>(spec[k].width) * spec[k].height; }
+
+  Pool pool() const {
+    Pool p;
+    p.count = count();
+    p.inst = inst_d_;
+    p.params = params_d_;
+    p.total_agents = total_agents_;
+    p.total_units = total_units_;
+    p.total_bins = total_bins_;
+    p.agent_block_inst = agent_block_d_;
+    p.unit_block_inst = unit_block_d_;
+    p.storage = cfg.storage;
+    p.pos64 = pos64() ? 1 : 0;
+    p.xcd_remap = (cfg.flags & SLM_FLAG_NO_XCD_REMAP) ? 0 : 1;
+    p.sort_cell_log2 = cfg.sort_cell_log2;
+    return p;
+  }
+
+  // ---- stepping -----------------------------------------------------------
+  int step(uint32_t n) {
+    uint32_t chunk = cfg.graph_chunk;
+    while (n > 0) {
+      if (chunk > 0 && n >= chunk) {
+        SLM_TRY(run_chunk_graph());
+        n -= chunk;
+      } else {
+        enqueue_step(sort_due(steps_done));
+        steps_done++;
+        n--;
       }
-}
-
-int alloc_trail(slm_sim* s) {
-  for (int i = 0; i < 2; i++) {
-    int rc = dev_alloc(s, &s->trail[i], trail_bytes(s));
-    if (rc) return rc;
-    SLM_HIP(hipMemsetAsync(s->trail[i], 0, trail_bytes(s), s->stream));
-  }
-  s->cur = 0;
-  return SLM_OK;
-}
-
-void free_trail(slm_sim* s) {
-  for (auto& t : s->trail) {
-    dev_free(s, t, trail_bytes(s));
-    t = nullptr;
-  }
-}
-
-int alloc_agents(slm_sim* s) {
-  uint64_t n = s->cfg.agent_count;
-  int sets = sorting(s) ? 2 : 1;
-  for (int k = 0; k < sets; k++) {
-    AgentSoA& a = s->agents[k];
-    int rc;
-    if ((rc = dev_alloc(s, reinterpret_cast<void**>(&a.x), n * 4))) return rc;
-    if ((rc = dev_alloc(s, reinterpret_cast<void**>(&a.y), n * 4))) return rc;
-    if ((rc = dev_alloc(s, reinterpret_cast<void**>(&a.th), n * 4))) return rc;
-    if ((rc = dev_alloc(s, reinterpret_cast<void**>(&a.v), n * 4))) return rc;
-    if (sorting(s) && (rc = dev_alloc(s, reinterpret_cast<void**>(&a.id), n * 4))) return rc;
-  }
-  if (sorting(s)) {
-    int rc;
-    if ((rc = dev_alloc(s, reinterpret_cast<void**>(&s->sort.key), n * 4))) return rc;
-    if ((rc = dev_alloc(s, reinterpret_cast<void**>(&s->sort.rank), n * 4))) return rc;
-  }
-  s->acur = 0;
-  return SLM_OK;
-}
-
-void free_agents(slm_sim* s) {
-  uint64_t n = s->cfg.agent_count;
-  for (auto& a : s->agents) {
-    dev_free(s, a.x, n * 4);
-    dev_free(s, a.y, n * 4);
-    dev_free(s, a.th, n * 4);
-    dev_free(s, a.v, n * 4);
-    dev_free(s, a.id, n * 4);
-    a = AgentSoA{};
-  }
-  dev_free(s, s->sort.key, n * 4);
-  dev_free(s, s->sort.rank, n * 4);
-  s->sort.key = s->sort.rank = nullptr;
-}
-
-int alloc_bins(slm_sim* s) {
-  if (!sorting(s)) return SLM_OK;
-  uint32_t cl = s->cfg.sort_cell_log2;
-  while (sort_bins_for(s->cfg.width, s->cfg.height, cl) > kMaxSortBins) cl++;
-  s->sort.cell_log2 = cl;
-  s->sort.nbins = sort_bins_for(s->cfg.width, s->cfg.height, cl);
-  int rc;
-  if ((rc = dev_alloc(s, reinterpret_cast<void**>(&s->sort.count), s->sort.nbins * 4ull))) return rc;
-  if ((rc = dev_alloc(s, reinterpret_cast<void**>(&s->sort.offset), s->sort.nbins * 4ull))) return rc;
-  if ((rc = dev_alloc(s, reinterpret_cast<void**>(&s->sort.block_sums),
-                      sort_block_sums_for(s->sort.nbins) * 4ull)))
-    return rc;
-  return SLM_OK;
-}
-
-void free_bins(slm_sim* s) {
-  dev_free(s, s->sort.count, s->sort.nbins * 4ull);
-  dev_free(s, s->sort.offset, s->sort.nbins * 4ull);
-  dev_free(s, s->sort.block_sums, sort_block_sums_for(s->sort.nbins) * 4ull);
-  s->sort.count = s->sort.offset = s->sort.block_sums = nullptr;
-}
-
-void init_agents(slm_sim* s) {
-  launch_init_agents(s->agents[s->acur], static_cast<uint32_t>(s->cfg.agent_count), s->cfg.seed,
-                     s->cfg.width, s->cfg.height, s->params.agent_speed_min,
-                     s->params.agent_speed_max, s->stream);
-}
-
-// Enqueue one step: optional re-sort, agents, trail pass.
-void enqueue_step(slm_sim* s, bool do_sort) {
-  StepGeom g = geom(s);
-  if (do_sort && sorting(s) && g.agents > 0) {
-    launch_morton_sort(g, s->agents[s->acur], s->agents[s->acur ^ 1], s->sort, s->stream);
-    s->acur ^= 1;
-  }
-  launch_agents(g, s->agents[s->acur], s->trail[s->cur], s->d_params, s->stream);
-  launch_trail(g, s->trail[s->cur], s->trail[s->cur ^ 1], s->d_params, s->stream);
-  s->cur ^= 1;
-}
-
-bool sort_due(const slm_sim* s, uint64_t step_index) {
-  return sorting(s) && step_index % s->cfg.resort_interval == 0;
-}
-
-int run_chunk_graph(slm_sim* s) {
-  uint32_t chunk = s->cfg.graph_chunk;
-  Graph& g = s->graphs[s->cur][s->acur];
-  int cur0 = s->cur, acur0 = s->acur;
-  if (!g.exec) {
-    hipGraph_t graph = nullptr;
-    SLM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    for (uint32_t j = 0; j < chunk; j++) enqueue_step(s, sorting(s) && j % s->cfg.resort_interval == 0);
-    hipError_t e = hipStreamEndCapture(s->stream, &graph);
-    int cur1 = s->cur, acur1 = s->acur;
-    s->cur = cur0;
-    s->acur = acur0;
-    if (e != hipSuccess) return fail(SLM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-      g.exec = nullptr;
-      return fail(SLM_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
     }
-    (void)cur1;
-    (void)acur1;
+    SLM_HIP(hipGetLastError());
+    return SLM_OK;
   }
-  SLM_HIP(hipGraphLaunch(g.exec, s->stream));
-  // replay the parity changes the captured steps make
-  for (uint32_t j = 0; j < chunk; j++) {
-    if (sorting(s) && j % s->cfg.resort_interval == 0 && s->cfg.agent_count > 0) s->acur ^= 1;
-    s->cur ^= 1;
+
+  int step_timed(uint32_t n, slm_timing* out) {
+    std::memset(out, 0, sizeof *out);
+    hipEvent_t ev[5];
+    for (auto& e : ev) SLM_HIP(hipEventCreate(&e));
+    int rc = SLM_OK;
+    Pool p = pool();
+    for (uint32_t i = 0; i < n && rc == SLM_OK; i++) {
+      bool sort = sort_due(steps_done) && total_agents_ > 0;
+      if (sort) {
+        (void)hipEventRecord(ev[0], stream);
+        launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
+        (void)hipEventRecord(ev[1], stream);
+        acur_ ^= 1;
+      }
+      (void)hipEventRecord(ev[2], stream);
+      launch_agents(p, agents_[acur_], trail_[cur_], stream);
+      (void)hipEventRecord(ev[3], stream);
+      launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
+      (void)hipEventRecord(ev[4], stream);
+      cur_ ^= 1;
+      steps_done++;
+      hipError_t e = hipEventSynchronize(ev[4]);
+      if (e != hipSuccess) {
+        rc = fail(SLM_ERR_HIP, "step failed: %s", hipGetErrorString(e));
+        break;
+      }
+      float ms = 0.0f;
+      if (sort && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
+        out->sort_ms += ms;
+        out->sort_launches++;
+      }
+      if (total_agents_ > 0 && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) {
+        out->agents_ms += ms;
+        out->agents_launches++;
+      }
+      if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess) {
+        out->trail_ms += ms;
+        out->trail_launches++;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    SLM_HIP(hipGetLastError());
+    return SLM_OK;
   }
-  s->steps_done += chunk;
-  return SLM_OK;
-}
 
-int sync_stream(slm_sim* s) {
-  SLM_HIP(hipStreamSynchronize(s->stream));
-  return SLM_OK;
-}
+  int sync() {
+    SLM_HIP(hipStreamSynchronize(stream));
+    return SLM_OK;
+  }
 
-void destroy_all(slm_sim* s) {
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  drop_graphs(s);
-  for (void* p : s->allocations)
-    if (p) (void)hipFree(p);
-  s->allocations.clear();
-  if (s->stream) (void)hipStreamDestroy(s->stream);
-  delete s;
+  // ---- per-instance operations ---------------------------------------------
+  int set_params(size_t k, const slm_params& p) {
+    spec[k].params = p;
+    launch_set_params(params_d_ + k, make_dev_params(p), stream);
+    SLM_HIP(hipGetLastError());
+    return SLM_OK;
+  }
+
+  int clear(size_t k) {
+    SLM_HIP(hipMemsetAsync(trail_ptr(cur_, k), 0, cells(k) * storage_bytes(cfg.storage), stream));
+    return SLM_OK;
+  }
+
+  int reassign_speeds(size_t k, float smin, float smax) {
+    reassigns[k]++;
+    uint64_t seed = spec[k].seed + reassigns[k] * 0xd1b54a32d192ed03ull;
+    launch_reassign_speeds(inst[k], agents_[acur_], seed, smin, smax, stream);
+    SLM_HIP(hipGetLastError());
+    return SLM_OK;
+  }
+
+  int read_agents(size_t k, float* out) {
+    uint64_t n = spec[k].agents;
+    if (n == 0) return SLM_OK;
+    void* staging = nullptr;
+    SLM_TRY(scratch_alloc(&staging, n * 16));
+    launch_pack_agents(inst[k], agents_[acur_], pos64() ? 1 : 0, static_cast<float*>(staging),
+                       stream);
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(out, staging, n * 16, hipMemcpyDeviceToHost);
+    (void)hipFree(staging);
+    if (e != hipSuccess) return fail(SLM_ERR_HIP, "read_agents failed: %s", hipGetErrorString(e));
+    return SLM_OK;
+  }
+
+  int write_agents(size_t k, const float* in) {
+    uint64_t n = spec[k].agents;
+    float fw = static_cast<float>(spec[k].width), fh = static_cast<float>(spec[k].height);
+    for (uint64_t i = 0; i < n; i++) {
+      const float* r = in + 4 * i;
+      if (!(r[0] >= 0.0f && r[0] <= fw && r[1] >= 0.0f && r[1] <= fh) || !std::isfinite(r[2]) ||
+          !std::isfinite(r[3]))
+        return fail(SLM_ERR_INVALID_ARGUMENT,
+                    "agent %llu: position must lie in [0,width]x[0,height], angle and speed finite",
+                    static_cast<unsigned long long>(i));
+    }
+    if (n == 0) return SLM_OK;
+    SLM_TRY(sync());
+    void* staging = nullptr;
+    SLM_TRY(scratch_alloc(&staging, n * 16));
+    hipError_t e = hipMemcpy(staging, in, n * 16, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_unpack_agents(inst[k], agents_[acur_], pos64() ? 1 : 0,
+                           static_cast<const float*>(staging), stream);
+      e = hipStreamSynchronize(stream);
+    }
+    (void)hipFree(staging);
+    if (e != hipSuccess) return fail(SLM_ERR_HIP, "write_agents failed: %s", hipGetErrorString(e));
+    return SLM_OK;
+  }
+
+  int read_trail(size_t k, float* out) {
+    SLM_TRY(sync());
+    uint64_t c = cells(k);
+    if (cfg.storage == SLM_STORAGE_F32) {
+      SLM_HIP(hipMemcpy(out, trail_ptr(cur_, k), c * 4, hipMemcpyDeviceToHost));
+      return SLM_OK;
+    }
+    void* staging = nullptr;
+    SLM_TRY(scratch_alloc(&staging, c * 4));
+    launch_decode_trail(trail_ptr(cur_, k), cfg.storage, c, static_cast<float*>(staging), stream);
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(out, staging, c * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(staging);
+    if (e != hipSuccess) return fail(SLM_ERR_HIP, "read_trail failed: %s", hipGetErrorString(e));
+    return SLM_OK;
+  }
+
+  int write_trail(size_t k, const float* in) {
+    uint64_t c = cells(k);
+    std::vector<float> clean(in, in + c);
+    for (uint64_t i = 0; i < c; i++) {
+      if (!(clean[i] >= 0.0f) || !std::isfinite(clean[i]))
+        return fail(SLM_ERR_INVALID_ARGUMENT, "trail cell %llu must be finite and >= 0",
+                    static_cast<unsigned long long>(i));
+      if (clean[i] == 0.0f) clean[i] = 0.0f;  // -0 -> +0: the sign bit is the deposit mark
+    }
+    SLM_TRY(sync());
+    if (cfg.storage == SLM_STORAGE_F32) {
+      SLM_HIP(hipMemcpy(trail_ptr(cur_, k), clean.data(), c * 4, hipMemcpyHostToDevice));
+      return SLM_OK;
+    }
+    void* staging = nullptr;
+    SLM_TRY(scratch_alloc(&staging, c * 4));
+    hipError_t e = hipMemcpy(staging, clean.data(), c * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_encode_trail(static_cast<const float*>(staging), cfg.storage, c, trail_ptr(cur_, k),
+                          stream);
+      e = hipStreamSynchronize(stream);
+    }
+    (void)hipFree(staging);
+    if (e != hipSuccess) return fail(SLM_ERR_HIP, "write_trail failed: %s", hipGetErrorString(e));
+    return SLM_OK;
+  }
+
+  int total_mass(double* out) {
+    launch_total_mass(pool(), trail_[cur_], mass_partials_, mass_out_, stream);
+    SLM_HIP(hipMemcpyAsync(out, mass_out_, count() * sizeof(double), hipMemcpyDeviceToHost,
+                           stream));
+    return sync();
+  }
+
+  // ---- single-simulation reshaping ----------------------------------------
+  int resize(uint32_t w, uint32_t h) {  // pool of one
+    SLM_TRY(sync());
+    drop_graphs();
+    float sx = static_cast<float>(w) / static_cast<float>(spec[0].width);
+    float sy = static_cast<float>(h) / static_cast<float>(spec[0].height);
+    launch_scale_positions(inst[0], agents_[acur_], pos64() ? 1 : 0, sx, sy, stream);
+    SLM_TRY(sync());
+    spec[0].width = w;
+    spec[0].height = h;
+    free_trail_layout();
+    return layout_trail();
+  }
+
+  int set_agent_count(uint64_t n) {  // pool of one
+    SLM_TRY(sync());
+    drop_graphs();
+    free_agent_layout();
+    spec[0].agents = n;
+    SLM_TRY(layout_agents());
+    SLM_TRY(layout_trail_tables());
+    init_agents(0);
+    return sync_checked("agent re-initialisation");
+  }
+
+ private:
+  Instance* inst_d_ = nullptr;
+  DevParams* params_d_ = nullptr;
+  uint32_t* agent_block_d_ = nullptr;
+  uint32_t* unit_block_d_ = nullptr;
+  void* trail_[2] = {nullptr, nullptr};
+  int cur_ = 0;
+  AgentSoA agents_[2] = {};
+  int acur_ = 0;
+  SortScratch sort_{};
+  uint32_t* scan_levels_ = nullptr;
+  double* mass_partials_ = nullptr;
+  double* mass_out_ = nullptr;
+  uint32_t total_agents_ = 0, total_units_ = 0, total_bins_ = 0;
+  uint64_t trail_elems_ = 0;
+  hipGraphExec_t graphs_[2][2] = {};
+  std::vector<std::pair<void*, size_t>> allocs_;
+
+  int dev_alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      (void)hipGetLastError();
+      return fail(e == hipErrorOutOfMemory ? SLM_ERR_OUT_OF_MEMORY : SLM_ERR_HIP,
+                  "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    allocs_.push_back({*p, bytes});
+    device_bytes += bytes;
+    return SLM_OK;
+  }
+
+  template <class T>
+  int dev_alloc(T** p, size_t bytes) {
+    return dev_alloc(reinterpret_cast<void**>(p), bytes);
+  }
+
+  void dev_free(void* p) {
+    if (!p) return;
+    for (auto& a : allocs_)
+      if (a.first == p) {
+        device_bytes -= a.second;
+        a.first = nullptr;
+      }
+    (void)hipFree(p);
+  }
+
+  // short-lived staging buffers (not counted in device_bytes)
+  int scratch_alloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      (void)hipGetLastError();
+      return fail(e == hipErrorOutOfMemory ? SLM_ERR_OUT_OF_MEMORY : SLM_ERR_HIP,
+                  "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return SLM_OK;
+  }
+
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    for (auto& a : allocs_)
+      if (a.first) (void)hipFree(a.first);
+    allocs_.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  int sync_checked(const char* what) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return fail(SLM_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+    return SLM_OK;
+  }
+
+  void* trail_ptr(int buf, size_t k) const {
+    return static_cast<char*>(trail_[buf]) + inst[k].trail_off * storage_bytes(cfg.storage);
+  }
+
+  int alloc_const() {
+    inst.assign(spec.size(), Instance{});
+    SLM_TRY(dev_alloc(&inst_d_, spec.size() * sizeof(Instance)));
+    SLM_TRY(dev_alloc(&params_d_, spec.size() * sizeof(DevParams)));
+    SLM_TRY(dev_alloc(&mass_partials_, spec.size() * kMassBlocks * sizeof(double)));
+    SLM_TRY(dev_alloc(&mass_out_, spec.size() * sizeof(double)));
+    return SLM_OK;
+  }
+
+  int upload_params_all() {
+    std::vector<DevParams> dp(spec.size());
+    for (size_t k = 0; k < spec.size(); k++) dp[k] = make_dev_params(spec[k].params);
+    SLM_HIP(hipMemcpy(params_d_, dp.data(), dp.size() * sizeof(DevParams), hipMemcpyHostToDevice));
+    return SLM_OK;
+  }
+
+  // Agent slots, agent arrays, re-sort per-slot scratch and the agent block map.
+  int layout_agents() {
+    uint64_t off = 0;
+    for (size_t k = 0; k < spec.size(); k++) {
+      inst[k].agent_off = static_cast<uint32_t>(off);
+      inst[k].agent_count = static_cast<uint32_t>(spec[k].agents);
+      off += spec[k].agents;
+    }
+    total_agents_ = static_cast<uint32_t>(off);
+    uint64_t n = off;
+    int sets = sorting() ? 2 : 1;
+    for (int s = 0; s < sets; s++) {
+      AgentSoA& a = agents_[s];
+      if (pos64()) {
+        SLM_TRY(dev_alloc(&a.x64, n * 8));
+        SLM_TRY(dev_alloc(&a.y64, n * 8));
+      } else {
+        SLM_TRY(dev_alloc(&a.x, n * 4));
+        SLM_TRY(dev_alloc(&a.y, n * 4));
+      }
+      SLM_TRY(dev_alloc(&a.th, n * 4));
+      SLM_TRY(dev_alloc(&a.v, n * 4));
+      if (sorting()) SLM_TRY(dev_alloc(&a.id, n * 4));
+    }
+    if (sorting()) {
+      SLM_TRY(dev_alloc(&sort_.key, n * 4));
+      SLM_TRY(dev_alloc(&sort_.rank, n * 4));
+    }
+    acur_ = 0;
+    // block -> first instance whose slots reach into the block
+    uint32_t nb = blocks_for(n, kAgentBlock);
+    std::vector<uint32_t> map(nb ? nb : 1, 0);
+    size_t k = 0;
+    for (uint32_t b = 0; b < nb; b++) {
+      uint64_t first = static_cast<uint64_t>(b) * kAgentBlock;
+      while (k + 1 < spec.size() &&
+             static_cast<uint64_t>(inst[k].agent_off) + inst[k].agent_count <= first)
+        k++;
+      map[b] = static_cast<uint32_t>(k);
+    }
+    SLM_TRY(dev_alloc(&agent_block_d_, map.size() * 4));
+    SLM_HIP(hipMemcpy(agent_block_d_, map.data(), map.size() * 4, hipMemcpyHostToDevice));
+    return SLM_OK;
+  }
+
+  void free_agent_layout() {
+    for (auto& a : agents_) {
+      dev_free(a.x);
+      dev_free(a.y);
+      dev_free(a.x64);
+      dev_free(a.y64);
+      dev_free(a.th);
+      dev_free(a.v);
+      dev_free(a.id);
+      a = AgentSoA{};
+    }
+    dev_free(sort_.key);
+    dev_free(sort_.rank);
+    sort_.key = sort_.rank = nullptr;
+    dev_free(agent_block_d_);
+    agent_block_d_ = nullptr;
+  }
+
+  // Trail buffers, trail-pass units, re-sort bins and their tables.
+  int layout_trail() {
+    uint64_t off = 0, total_cells = 0;
+    for (size_t k = 0; k < spec.size(); k++) total_cells += cells(k);
+    for (size_t k = 0; k < spec.size(); k++) {
+      Instance& in = inst[k];
+      in.width = spec[k].width;
+      in.height = spec[k].height;
+      in.trail_off = off;
+      off += (cells(k) + 127u) & ~uint64_t(127u);  // 256-byte aligned starts
+    }
+    trail_elems_ = off + 128u;  // slack: tap pairs and 32-bit halo words stay inside
+    for (int b = 0; b < 2; b++) {
+      SLM_TRY(dev_alloc(&trail_[b], trail_elems_ * storage_bytes(cfg.storage)));
+      SLM_HIP(hipMemsetAsync(trail_[b], 0, trail_elems_ * storage_bytes(cfg.storage), stream));
+    }
+    cur_ = 0;
+    (void)total_cells;
+    return layout_trail_tables();
+  }
+
+  // Units, bins and block maps (also redone when the agent layout changes).
+  int layout_trail_tables() {
+    uint64_t total_cells = 0;
+    for (size_t k = 0; k < spec.size(); k++) total_cells += cells(k);
+    uint64_t units = 0, bins = 0;
+    for (size_t k = 0; k < spec.size(); k++) {
+      Instance& in = inst[k];
+      uint32_t target = static_cast<uint32_t>(
+          std::max<uint64_t>(1, kTargetWaves * cells(k) / std::max<uint64_t>(1, total_cells)));
+      plan_trail(in, cfg.storage, target);
+      in.unit_off = static_cast<uint32_t>(units);
+      units += in.units;
+      uint32_t cl = cfg.sort_cell_log2;
+      plan_bins(in, cl);
+      in.bin_off = static_cast<uint32_t>(bins);
+      bins += in.nbins;
+    }
+    if (units >= (1ull << 32)) return fail(SLM_ERR_INVALID_ARGUMENT, "too many trail units");
+    if (sorting() && bins > kMaxTotalBins)
+      return fail(SLM_ERR_INVALID_ARGUMENT, "too many re-sort bins: raise sort_cell_log2");
+    total_units_ = static_cast<uint32_t>(units);
+    total_bins_ = sorting() ? static_cast<uint32_t>(bins) : 0;
+    SLM_HIP(hipMemcpy(inst_d_, inst.data(), inst.size() * sizeof(Instance), hipMemcpyHostToDevice));
+
+    dev_free(unit_block_d_);
+    uint32_t nb = blocks_for(units, kTrailWavesPerBlock);
+    std::vector<uint32_t> map(nb ? nb : 1, 0);
+    size_t k = 0;
+    for (uint32_t b = 0; b < nb; b++) {
+      uint64_t first = static_cast<uint64_t>(b) * kTrailWavesPerBlock;
+      while (k + 1 < spec.size() && static_cast<uint64_t>(inst[k].unit_off) + inst[k].units <= first)
+        k++;
+      map[b] = static_cast<uint32_t>(k);
+    }
+    SLM_TRY(dev_alloc(&unit_block_d_, map.size() * 4));
+    SLM_HIP(hipMemcpy(unit_block_d_, map.data(), map.size() * 4, hipMemcpyHostToDevice));
+
+    dev_free(sort_.count);
+    dev_free(sort_.offset);
+    dev_free(scan_levels_);
+    sort_.count = sort_.offset = scan_levels_ = nullptr;
+    if (sorting()) {
+      SLM_TRY(dev_alloc(&sort_.count, total_bins_ * 4ull));
+      SLM_TRY(dev_alloc(&sort_.offset, total_bins_ * 4ull));
+      uint32_t lv = plan_scan(sort_, total_bins_);
+      SLM_TRY(dev_alloc(&scan_levels_, lv * 4ull));
+      uint32_t at = 0;
+      for (int l = 0; l < 3; l++) {
+        sort_.level[l] = l < sort_.levels ? scan_levels_ + at : nullptr;
+        if (l < sort_.levels) at += sort_.level_len[l];
+      }
+    }
+    return SLM_OK;
+  }
+
+  void free_trail_layout() {
+    dev_free(trail_[0]);
+    dev_free(trail_[1]);
+    trail_[0] = trail_[1] = nullptr;
+  }
+
+  void init_agents(size_t k) {
+    launch_init_agents(inst[k], agents_[acur_], pos64() ? 1 : 0, spec[k].seed,
+                       spec[k].params.agent_speed_min, spec[k].params.agent_speed_max, stream);
+  }
+
+  bool sort_due(uint64_t step_index) const {
+    return sorting() && step_index % cfg.resort_interval == 0;
+  }
+
+  void enqueue_step(bool do_sort) {
+    Pool p = pool();
+    if (do_sort && sorting() && total_agents_ > 0) {
+      launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
+      acur_ ^= 1;
+    }
+    launch_agents(p, agents_[acur_], trail_[cur_], stream);
+    launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
+    cur_ ^= 1;
+  }
+
+  bool chunk_sorts(uint32_t j) const {
+    return sorting() && total_agents_ > 0 && j % cfg.resort_interval == 0;
+  }
+
+  int run_chunk_graph() {
+    uint32_t chunk = cfg.graph_chunk;
+    hipGraphExec_t& exec = graphs_[cur_][acur_];
+    if (!exec) {
+      int cur0 = cur_, acur0 = acur_;
+      hipGraph_t graph = nullptr;
+      SLM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      for (uint32_t j = 0; j < chunk; j++) enqueue_step(chunk_sorts(j));
+      hipError_t e = hipStreamEndCapture(stream, &graph);
+      cur_ = cur0;
+      acur_ = acur0;
+      if (e != hipSuccess) return fail(SLM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        exec = nullptr;
+        return fail(SLM_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+      }
+    }
+    SLM_HIP(hipGraphLaunch(exec, stream));
+    for (uint32_t j = 0; j < chunk; j++) {  // the buffer swaps the captured steps make
+      if (chunk_sorts(j)) acur_ ^= 1;
+      cur_ ^= 1;
+    }
+    steps_done += chunk;
+    return SLM_OK;
+  }
+
+  void drop_graphs() {
+    for (auto& row : graphs_)
+      for (auto& g : row)
+        if (g) {
+          (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
+  }
+};
+
+int make_engine(const EngineConfig& ec, std::vector<InstanceSpec> specs, Engine** out) {
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(SLM_ERR_NO_DEVICE, "no HIP device: %s", hipGetErrorString(e));
+  }
+  if (ec.device < 0 || ec.device >= ndev)
+    return fail(SLM_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", ec.device, ndev);
+  DeviceGuard guard(ec.device);
+  Engine* eng = new (std::nothrow) Engine();
+  if (!eng) return fail(SLM_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int rc = eng->create(ec, std::move(specs));
+  if (rc) {
+    delete eng;
+    return rc;
+  }
+  *out = eng;
+  return SLM_OK;
 }
 
 }  // namespace
+
+struct slm_sim {
+  Engine* eng;
+  slm_config cfg;
+};
+
+struct slm_batch {
+  Engine* eng;
+  slm_batch_config cfg;
+};
 
 extern "C" {
 
@@ -352,332 +761,308 @@ int slm_params_default(slm_params* p) {
   return SLM_OK;
 }
 
+// ---- single simulation ------------------------------------------------------
+
 int slm_create(const slm_config* cfg, const slm_params* params, slm_sim** out) {
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
   if (!cfg) return fail(SLM_ERR_INVALID_ARGUMENT, "cfg is NULL");
-  int rc = check_params(params);
-  if (rc) return rc;
-  if ((rc = check_grid(cfg->width, cfg->height))) return rc;
+  SLM_TRY(check_params(params));
+  SLM_TRY(check_grid(cfg->width, cfg->height));
   if (cfg->agent_count > kMaxAgents)
     return fail(SLM_ERR_INVALID_ARGUMENT, "agent_count must be < 2^32 - 256");
-  if (cfg->storage < SLM_STORAGE_F32 || cfg->storage > SLM_STORAGE_F16)
-    return fail(SLM_ERR_INVALID_ARGUMENT, "unknown storage %d", cfg->storage);
-  if (cfg->sort_cell_log2 > 12) return fail(SLM_ERR_INVALID_ARGUMENT, "sort_cell_log2 must be <= 12");
-  if (cfg->graph_chunk > 100000) return fail(SLM_ERR_INVALID_ARGUMENT, "graph_chunk must be <= 100000");
-  if (cfg->flags & ~SLM_FLAG_NO_XCD_REMAP) return fail(SLM_ERR_INVALID_ARGUMENT, "unknown flags");
-  for (uint32_t r : cfg->reserved)
-    if (r) return fail(SLM_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
-  int ndev = 0;
-  if ((rc = slm_device_count(&ndev))) return rc;
-  if (cfg->device < 0 || cfg->device >= ndev)
-    return fail(SLM_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", cfg->device, ndev);
-
-  DeviceGuard guard(cfg->device);
-  slm_sim* s = new (std::nothrow) slm_sim();
+  EngineConfig ec;
+  ec.storage = cfg->storage;
+  ec.device = cfg->device;
+  ec.resort_interval = cfg->resort_interval;
+  ec.graph_chunk = cfg->graph_chunk;
+  ec.sort_cell_log2 = cfg->sort_cell_log2;
+  ec.flags = cfg->flags;
+  SLM_TRY(check_engine_config(ec, cfg->reserved, 6));
+  std::vector<InstanceSpec> specs{{cfg->width, cfg->height, cfg->agent_count, cfg->seed, *params}};
+  slm_sim* s = new (std::nothrow) slm_sim{nullptr, *cfg};
   if (!s) return fail(SLM_ERR_OUT_OF_MEMORY, "host allocation failed");
-  s->cfg = *cfg;
-  s->params = *params;
-  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
+  int rc = make_engine(ec, std::move(specs), &s->eng);
+  if (rc) {
     delete s;
-    return fail(SLM_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
-  }
-  if ((rc = dev_alloc(s, reinterpret_cast<void**>(&s->d_params), sizeof(DevParams))) ||
-      (rc = dev_alloc(s, reinterpret_cast<void**>(&s->mass_partials), kMassBlocks * sizeof(double))) ||
-      (rc = dev_alloc(s, reinterpret_cast<void**>(&s->mass_out), sizeof(double))) ||
-      (rc = alloc_trail(s)) || (rc = alloc_agents(s)) || (rc = alloc_bins(s))) {
-    destroy_all(s);
     return rc;
-  }
-  launch_set_params(s->d_params, make_dev_params(s->params), s->stream);
-  init_agents(s);
-  e = hipStreamSynchronize(s->stream);
-  if (e == hipSuccess) e = hipGetLastError();
-  if (e != hipSuccess) {
-    destroy_all(s);
-    return fail(SLM_ERR_HIP, "initialisation failed: %s", hipGetErrorString(e));
   }
   *out = s;
   return SLM_OK;
 }
 
+#define SIM_GUARD(s)                                                      \
+  if (!(s) || !(s)->eng) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL"); \
+  DeviceGuard guard_((s)->eng->cfg.device)
+
 int slm_destroy(slm_sim* s) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
-  DeviceGuard guard(s->cfg.device);
-  destroy_all(s);
+  SIM_GUARD(s);
+  delete s->eng;
+  delete s;
   return SLM_OK;
 }
 
 int slm_set_params(slm_sim* s, const slm_params* p) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
-  int rc = check_params(p);
-  if (rc) return rc;
-  DeviceGuard guard(s->cfg.device);
-  s->params = *p;
-  launch_set_params(s->d_params, make_dev_params(*p), s->stream);
-  SLM_HIP(hipGetLastError());
-  return SLM_OK;
+  SIM_GUARD(s);
+  SLM_TRY(check_params(p));
+  return s->eng->set_params(0, *p);
 }
 
 int slm_get_params(const slm_sim* s, slm_params* out) {
-  if (!s || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  *out = s->params;
+  if (!s || !s->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = s->eng->spec[0].params;
   return SLM_OK;
 }
 
 int slm_step(slm_sim* s, uint32_t n) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
-  DeviceGuard guard(s->cfg.device);
-  uint32_t chunk = s->cfg.graph_chunk;
-  while (n > 0) {
-    if (chunk > 0 && n >= chunk) {
-      int rc = run_chunk_graph(s);
-      if (rc) return rc;
-      n -= chunk;
-    } else {
-      enqueue_step(s, sort_due(s, s->steps_done));
-      s->steps_done++;
-      n--;
-    }
-  }
-  SLM_HIP(hipGetLastError());
-  return SLM_OK;
+  SIM_GUARD(s);
+  return s->eng->step(n);
 }
 
 int slm_step_timed(slm_sim* s, uint32_t n, slm_timing* out) {
-  if (!s || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  DeviceGuard guard(s->cfg.device);
-  std::memset(out, 0, sizeof *out);
-  hipEvent_t ev[6];
-  for (auto& e : ev) SLM_HIP(hipEventCreate(&e));
-  int rc = SLM_OK;
-  StepGeom g = geom(s);
-  for (uint32_t i = 0; i < n && rc == SLM_OK; i++) {
-    bool sort = sort_due(s, s->steps_done) && g.agents > 0;
-    if (sort) {
-      (void)hipEventRecord(ev[0], s->stream);
-      launch_morton_sort(g, s->agents[s->acur], s->agents[s->acur ^ 1], s->sort, s->stream);
-      (void)hipEventRecord(ev[1], s->stream);
-      s->acur ^= 1;
-    }
-    (void)hipEventRecord(ev[2], s->stream);
-    launch_agents(g, s->agents[s->acur], s->trail[s->cur], s->d_params, s->stream);
-    (void)hipEventRecord(ev[3], s->stream);
-    launch_trail(g, s->trail[s->cur], s->trail[s->cur ^ 1], s->d_params, s->stream);
-    (void)hipEventRecord(ev[4], s->stream);
-    s->cur ^= 1;
-    s->steps_done++;
-    hipError_t e = hipEventSynchronize(ev[4]);
-    if (e != hipSuccess) {
-      rc = fail(SLM_ERR_HIP, "step failed: %s", hipGetErrorString(e));
-      break;
-    }
-    float ms = 0.0f;
-    if (sort && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
-      out->sort_ms += ms;
-      out->sort_launches++;
-    }
-    if (g.agents > 0 && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) {
-      out->agents_ms += ms;
-      out->agents_launches++;
-    }
-    if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess) {
-      out->trail_ms += ms;
-      out->trail_launches++;
-    }
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  if (rc) return rc;
-  SLM_HIP(hipGetLastError());
-  return SLM_OK;
+  SIM_GUARD(s);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  return s->eng->step_timed(n, out);
 }
 
 int slm_synchronize(slm_sim* s) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
-  DeviceGuard guard(s->cfg.device);
-  return sync_stream(s);
+  SIM_GUARD(s);
+  return s->eng->sync();
 }
 
 int slm_clear(slm_sim* s) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
-  DeviceGuard guard(s->cfg.device);
-  SLM_HIP(hipMemsetAsync(s->trail[s->cur], 0, trail_bytes(s), s->stream));
-  return SLM_OK;
+  SIM_GUARD(s);
+  return s->eng->clear(0);
 }
 
 int slm_resize(slm_sim* s, uint32_t w, uint32_t h) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
-  int rc = check_grid(w, h);
-  if (rc) return rc;
-  DeviceGuard guard(s->cfg.device);
-  if ((rc = sync_stream(s))) return rc;
-  drop_graphs(s);
-  // src/main.rs:993-994: ratio computed in f32, then multiplied in
-  float sx = static_cast<float>(w) / static_cast<float>(s->cfg.width);
-  float sy = static_cast<float>(h) / static_cast<float>(s->cfg.height);
-  launch_scale_positions(s->agents[s->acur], static_cast<uint32_t>(s->cfg.agent_count), sx, sy,
-                         s->stream);
-  if ((rc = sync_stream(s))) return rc;
-  free_trail(s);
-  free_bins(s);
+  SIM_GUARD(s);
+  SLM_TRY(check_grid(w, h));
+  SLM_TRY(s->eng->resize(w, h));
   s->cfg.width = w;
   s->cfg.height = h;
-  if ((rc = alloc_trail(s)) || (rc = alloc_bins(s))) return rc;
-  return sync_stream(s);
+  return SLM_OK;
 }
 
 int slm_set_agent_count(slm_sim* s, uint64_t n) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
+  SIM_GUARD(s);
   if (n > kMaxAgents) return fail(SLM_ERR_INVALID_ARGUMENT, "agent_count must be < 2^32 - 256");
-  DeviceGuard guard(s->cfg.device);
-  int rc = sync_stream(s);
-  if (rc) return rc;
-  drop_graphs(s);
-  free_agents(s);
+  SLM_TRY(s->eng->set_agent_count(n));
   s->cfg.agent_count = n;
-  if ((rc = alloc_agents(s))) return rc;
-  init_agents(s);
-  return sync_stream(s);
+  return SLM_OK;
 }
 
 int slm_reassign_speeds(slm_sim* s, float smin, float smax) {
-  if (!s) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL");
+  SIM_GUARD(s);
   if (!std::isfinite(smin) || !std::isfinite(smax))
     return fail(SLM_ERR_INVALID_ARGUMENT, "speeds must be finite");
-  DeviceGuard guard(s->cfg.device);
-  s->reassign_calls++;
-  uint64_t seed = s->cfg.seed + s->reassign_calls * 0xd1b54a32d192ed03ull;
-  launch_reassign_speeds(s->agents[s->acur], static_cast<uint32_t>(s->cfg.agent_count), seed, smin,
-                         smax, s->stream);
-  SLM_HIP(hipGetLastError());
+  return s->eng->reassign_speeds(0, smin, smax);
+}
+
+static int check_n(uint64_t n, uint64_t want, const char* what) {
+  if (n != want)
+    return fail(SLM_ERR_INVALID_ARGUMENT, "%s: n (%llu) != expected (%llu)", what,
+                static_cast<unsigned long long>(n), static_cast<unsigned long long>(want));
   return SLM_OK;
 }
 
 int slm_read_agents(slm_sim* s, float* out, uint64_t n) {
-  if (!s || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (n != s->cfg.agent_count)
-    return fail(SLM_ERR_INVALID_ARGUMENT, "n (%llu) != agent count (%llu)",
-                static_cast<unsigned long long>(n),
-                static_cast<unsigned long long>(s->cfg.agent_count));
-  if (n == 0) return SLM_OK;
-  DeviceGuard guard(s->cfg.device);
-  void* staging = nullptr;
-  int rc = dev_alloc(s, &staging, n * 16);
-  if (rc) return rc;
-  launch_pack_agents(s->agents[s->acur], static_cast<uint32_t>(n), static_cast<float*>(staging),
-                     s->stream);
-  hipError_t e = hipStreamSynchronize(s->stream);
-  if (e == hipSuccess) e = hipMemcpy(out, staging, n * 16, hipMemcpyDeviceToHost);
-  dev_free(s, staging, n * 16);
-  if (e != hipSuccess) return fail(SLM_ERR_HIP, "read_agents failed: %s", hipGetErrorString(e));
-  return SLM_OK;
+  SIM_GUARD(s);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(n, s->eng->spec[0].agents, "read_agents"));
+  return s->eng->read_agents(0, out);
 }
 
 int slm_write_agents(slm_sim* s, const float* in, uint64_t n) {
-  if (!s || !in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (n != s->cfg.agent_count)
-    return fail(SLM_ERR_INVALID_ARGUMENT, "n (%llu) != agent count (%llu)",
-                static_cast<unsigned long long>(n),
-                static_cast<unsigned long long>(s->cfg.agent_count));
-  float fw = static_cast<float>(s->cfg.width), fh = static_cast<float>(s->cfg.height);
-  for (uint64_t i = 0; i < n; i++) {
-    const float* r = in + 4 * i;
-    if (!(r[0] >= 0.0f && r[0] <= fw && r[1] >= 0.0f && r[1] <= fh) || !std::isfinite(r[2]) ||
-        !std::isfinite(r[3]))
-      return fail(SLM_ERR_INVALID_ARGUMENT,
-                  "agent %llu: position must lie in [0,width]x[0,height], angle and speed finite",
-                  static_cast<unsigned long long>(i));
-  }
-  if (n == 0) return SLM_OK;
-  DeviceGuard guard(s->cfg.device);
-  int rc = sync_stream(s);
-  if (rc) return rc;
-  void* staging = nullptr;
-  if ((rc = dev_alloc(s, &staging, n * 16))) return rc;
-  hipError_t e = hipMemcpy(staging, in, n * 16, hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    launch_unpack_agents(s->agents[s->acur], static_cast<uint32_t>(n),
-                         static_cast<const float*>(staging), s->stream);
-    e = hipStreamSynchronize(s->stream);
-  }
-  dev_free(s, staging, n * 16);
-  if (e != hipSuccess) return fail(SLM_ERR_HIP, "write_agents failed: %s", hipGetErrorString(e));
-  return SLM_OK;
+  SIM_GUARD(s);
+  if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(n, s->eng->spec[0].agents, "write_agents"));
+  return s->eng->write_agents(0, in);
 }
 
 int slm_read_trail(slm_sim* s, float* out, uint64_t cells) {
-  if (!s || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (cells != cells_of(s)) return fail(SLM_ERR_INVALID_ARGUMENT, "cells != width*height");
-  DeviceGuard guard(s->cfg.device);
-  int rc = sync_stream(s);
-  if (rc) return rc;
-  if (s->cfg.storage == SLM_STORAGE_F32) {
-    SLM_HIP(hipMemcpy(out, s->trail[s->cur], cells * 4, hipMemcpyDeviceToHost));
-    return SLM_OK;
-  }
-  void* staging = nullptr;
-  if ((rc = dev_alloc(s, &staging, cells * 4))) return rc;
-  launch_decode_trail(s->trail[s->cur], s->cfg.storage, cells, static_cast<float*>(staging),
-                      s->stream);
-  hipError_t e = hipStreamSynchronize(s->stream);
-  if (e == hipSuccess) e = hipMemcpy(out, staging, cells * 4, hipMemcpyDeviceToHost);
-  dev_free(s, staging, cells * 4);
-  if (e != hipSuccess) return fail(SLM_ERR_HIP, "read_trail failed: %s", hipGetErrorString(e));
-  return SLM_OK;
+  SIM_GUARD(s);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(cells, s->eng->cells(0), "read_trail (cells != width*height)"));
+  return s->eng->read_trail(0, out);
 }
 
 int slm_write_trail(slm_sim* s, const float* in, uint64_t cells) {
-  if (!s || !in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (cells != cells_of(s)) return fail(SLM_ERR_INVALID_ARGUMENT, "cells != width*height");
-  std::vector<float> clean(in, in + cells);
-  for (uint64_t i = 0; i < cells; i++) {
-    if (!(clean[i] >= 0.0f) || !std::isfinite(clean[i]))
-      return fail(SLM_ERR_INVALID_ARGUMENT, "trail cell %llu must be finite and >= 0",
-                  static_cast<unsigned long long>(i));
-    if (clean[i] == 0.0f) clean[i] = 0.0f;  // -0 -> +0: the sign bit is the deposit mark
-  }
-  DeviceGuard guard(s->cfg.device);
-  int rc = sync_stream(s);
-  if (rc) return rc;
-  if (s->cfg.storage == SLM_STORAGE_F32) {
-    SLM_HIP(hipMemcpy(s->trail[s->cur], clean.data(), cells * 4, hipMemcpyHostToDevice));
-    return SLM_OK;
-  }
-  void* staging = nullptr;
-  if ((rc = dev_alloc(s, &staging, cells * 4))) return rc;
-  hipError_t e = hipMemcpy(staging, clean.data(), cells * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    launch_encode_trail(static_cast<const float*>(staging), s->cfg.storage, cells,
-                        s->trail[s->cur], s->stream);
-    e = hipStreamSynchronize(s->stream);
-  }
-  dev_free(s, staging, cells * 4);
-  if (e != hipSuccess) return fail(SLM_ERR_HIP, "write_trail failed: %s", hipGetErrorString(e));
-  return SLM_OK;
+  SIM_GUARD(s);
+  if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(cells, s->eng->cells(0), "write_trail (cells != width*height)"));
+  return s->eng->write_trail(0, in);
 }
 
 int slm_total_mass(slm_sim* s, double* out) {
-  if (!s || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  DeviceGuard guard(s->cfg.device);
-  launch_total_mass(s->trail[s->cur], s->cfg.storage, cells_of(s), s->mass_partials, s->mass_out,
-                    s->stream);
-  SLM_HIP(hipMemcpyAsync(out, s->mass_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  return sync_stream(s);
+  SIM_GUARD(s);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  return s->eng->total_mass(out);
 }
 
 int slm_get_info(const slm_sim* s, slm_info* out) {
-  if (!s || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!s || !s->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   std::memset(out, 0, sizeof *out);
-  out->width = s->cfg.width;
-  out->height = s->cfg.height;
-  out->agent_count = s->cfg.agent_count;
-  out->storage = s->cfg.storage;
-  out->device = s->cfg.device;
-  out->steps_done = s->steps_done;
-  out->device_bytes = s->device_bytes;
-  out->resort_interval = s->cfg.resort_interval;
-  out->graph_chunk = s->cfg.graph_chunk;
+  const Engine& e = *s->eng;
+  out->width = e.spec[0].width;
+  out->height = e.spec[0].height;
+  out->agent_count = e.spec[0].agents;
+  out->storage = e.cfg.storage;
+  out->device = e.cfg.device;
+  out->steps_done = e.steps_done;
+  out->device_bytes = e.device_bytes;
+  out->resort_interval = e.cfg.resort_interval;
+  out->graph_chunk = e.cfg.graph_chunk;
+  return SLM_OK;
+}
+
+// ---- batches ------------------------------------------------------------------
+
+int slm_batch_config_default(slm_batch_config* cfg) {
+  if (!cfg) return fail(SLM_ERR_INVALID_ARGUMENT, "cfg is NULL");
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->storage = SLM_STORAGE_F32;
+  cfg->graph_chunk = 100;
+  cfg->sort_cell_log2 = 4;
+  return SLM_OK;
+}
+
+int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts, uint32_t count,
+                     slm_batch** out) {
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (!cfg || !insts) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (count == 0) return fail(SLM_ERR_INVALID_ARGUMENT, "a batch needs at least one instance");
+  if (count > 65535) return fail(SLM_ERR_INVALID_ARGUMENT, "a batch holds at most 65535 instances");
+  EngineConfig ec;
+  ec.storage = cfg->storage;
+  ec.device = cfg->device;
+  ec.resort_interval = cfg->resort_interval;
+  ec.graph_chunk = cfg->graph_chunk;
+  ec.sort_cell_log2 = cfg->sort_cell_log2;
+  ec.flags = cfg->flags;
+  SLM_TRY(check_engine_config(ec, cfg->reserved, 6));
+  std::vector<InstanceSpec> specs;
+  specs.reserve(count);
+  uint64_t agents = 0, cells = 0;
+  for (uint32_t k = 0; k < count; k++) {
+    const slm_instance_desc& d = insts[k];
+    int rc = check_grid(d.width, d.height);
+    if (!rc) rc = check_params(&d.params);
+    if (rc) return fail(rc, "instance %u: %s", k, g_last_error.c_str());
+    agents += d.agent_count;
+    cells += static_cast<uint64_t>(d.width) * d.height;
+    specs.push_back({d.width, d.height, d.agent_count, d.seed, d.params});
+  }
+  if (agents > kMaxAgents) return fail(SLM_ERR_INVALID_ARGUMENT, "total agents must be < 2^32 - 256");
+  if (cells > kMaxPoolCells) return fail(SLM_ERR_INVALID_ARGUMENT, "total cells must be <= 2^40");
+  slm_batch* b = new (std::nothrow) slm_batch{nullptr, *cfg};
+  if (!b) return fail(SLM_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int rc = make_engine(ec, std::move(specs), &b->eng);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return SLM_OK;
+}
+
+#define BATCH_GUARD(b)                                                        \
+  if (!(b) || !(b)->eng) return fail(SLM_ERR_INVALID_ARGUMENT, "batch is NULL"); \
+  DeviceGuard guard_((b)->eng->cfg.device)
+
+#define BATCH_INDEX(b, k)                                                                \
+  if ((k) >= (b)->eng->count())                                                          \
+  return fail(SLM_ERR_INVALID_ARGUMENT, "instance %u out of range (%u instances)", (k), \
+              (b)->eng->count())
+
+int slm_batch_destroy(slm_batch* b) {
+  BATCH_GUARD(b);
+  delete b->eng;
+  delete b;
+  return SLM_OK;
+}
+
+int slm_batch_step(slm_batch* b, uint32_t n) {
+  BATCH_GUARD(b);
+  return b->eng->step(n);
+}
+
+int slm_batch_step_timed(slm_batch* b, uint32_t n, slm_timing* out) {
+  BATCH_GUARD(b);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  return b->eng->step_timed(n, out);
+}
+
+int slm_batch_synchronize(slm_batch* b) {
+  BATCH_GUARD(b);
+  return b->eng->sync();
+}
+
+int slm_batch_set_params(slm_batch* b, uint32_t k, const slm_params* p) {
+  BATCH_GUARD(b);
+  BATCH_INDEX(b, k);
+  SLM_TRY(check_params(p));
+  return b->eng->set_params(k, *p);
+}
+
+int slm_batch_clear(slm_batch* b, uint32_t k) {
+  BATCH_GUARD(b);
+  BATCH_INDEX(b, k);
+  return b->eng->clear(k);
+}
+
+int slm_batch_read_agents(slm_batch* b, uint32_t k, float* out, uint64_t n) {
+  BATCH_GUARD(b);
+  BATCH_INDEX(b, k);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(n, b->eng->spec[k].agents, "read_agents"));
+  return b->eng->read_agents(k, out);
+}
+
+int slm_batch_write_agents(slm_batch* b, uint32_t k, const float* in, uint64_t n) {
+  BATCH_GUARD(b);
+  BATCH_INDEX(b, k);
+  if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(n, b->eng->spec[k].agents, "write_agents"));
+  return b->eng->write_agents(k, in);
+}
+
+int slm_batch_read_trail(slm_batch* b, uint32_t k, float* out, uint64_t cells) {
+  BATCH_GUARD(b);
+  BATCH_INDEX(b, k);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(cells, b->eng->cells(k), "read_trail (cells != width*height)"));
+  return b->eng->read_trail(k, out);
+}
+
+int slm_batch_write_trail(slm_batch* b, uint32_t k, const float* in, uint64_t cells) {
+  BATCH_GUARD(b);
+  BATCH_INDEX(b, k);
+  if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(cells, b->eng->cells(k), "write_trail (cells != width*height)"));
+  return b->eng->write_trail(k, in);
+}
+
+int slm_batch_total_mass(slm_batch* b, double* out, uint32_t count) {
+  BATCH_GUARD(b);
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(count, b->eng->count(), "total_mass (count != instances)"));
+  return b->eng->total_mass(out);
+}
+
+int slm_batch_get_info(const slm_batch* b, slm_batch_info* out) {
+  if (!b || !b->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::memset(out, 0, sizeof *out);
+  const Engine& e = *b->eng;
+  out->count = e.count();
+  out->storage = e.cfg.storage;
+  for (size_t k = 0; k < e.spec.size(); k++) {
+    out->total_agents += e.spec[k].agents;
+    out->total_cells += e.cells(k);
+  }
+  out->steps_done = e.steps_done;
+  out->device_bytes = e.device_bytes;
   return SLM_OK;
 }
 

@@ -30,64 +30,101 @@ struct DevParams {
 
 DevParams make_dev_params(const slm_params& p);
 
-// Agents in structure-of-arrays form.  id[j] is the original index of the
-// agent stored at slot j (the jitter hash and slm_read_agents use it); it is
-// only maintained when re-sorting is enabled, otherwise id == nullptr and the
-// slot is the original index.
+// Agents in structure-of-arrays form.  Positions are f32 (x, y) or, in the
+// f64-position mode, f64 (x64, y64).  id[j] is the original (instance-local)
+// index of the agent stored at slot j; the jitter hash and slm_read_agents
+// use it.  It is only kept when re-sorting is enabled; otherwise id == nullptr
+// and the slot minus the instance's first slot is the original index.
 struct AgentSoA {
   float* x;
   float* y;
+  double* x64;
+  double* y64;
   float* th;
   float* v;
   uint32_t* id;
 };
 
-struct StepGeom {
-  uint32_t width;
-  uint32_t height;
-  uint32_t agents;
-  int storage;   // slm_storage
-  int xcd_remap;  // 1: group consecutive logical blocks on one XCD
+// One simulation inside a pool of several (a single simulation is a pool of
+// one).  Offsets index the pooled agent arrays (slots) and trail buffers
+// (elements).
+struct Instance {
+  uint32_t width, height;
+  uint32_t agent_off, agent_count;  // slots [agent_off, agent_off + agent_count)
+  uint64_t trail_off;               // first element of this instance's trail
+  uint32_t unit_off, units;         // trail-pass wave units [unit_off, unit_off + units)
+  uint32_t tile_rows, ntx;          // strip tiles (ntx > 0) or 64-cell units (ntx == 0)
+  uint32_t bin_off;                 // re-sort bins [bin_off, bin_off + nbins)
+  uint32_t nbins;
+  uint32_t nbx, nby;                // bins across / down
 };
 
+// A launchable description of a pool, with device copies of its tables.
+struct Pool {
+  uint32_t count;            // instances
+  const Instance* inst;      // device, count entries
+  const DevParams* params;   // device, count entries
+  uint32_t total_agents;     // slots
+  uint32_t total_units;      // trail-pass wave units
+  uint32_t total_bins;
+  const uint32_t* agent_block_inst;  // device: first instance of each agent block
+  const uint32_t* unit_block_inst;   // device: first instance of each trail block
+  int storage;    // slm_storage
+  int pos64;      // f64 positions
+  int xcd_remap;  // group consecutive logical blocks on one XCD
+  uint32_t sort_cell_log2;
+};
+
+constexpr int kAgentBlock = 256;
+constexpr int kTrailBlock = 256;
+constexpr int kTrailWavesPerBlock = kTrailBlock / 64;
+
+inline uint64_t storage_bytes(int storage) { return storage == SLM_STORAGE_F32 ? 4 : 2; }
+inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
+
+// Host-side layout of one instance's trail pass: fills tile_rows, ntx, units.
+void plan_trail(Instance& in, int storage, uint32_t target_waves);
+// Host-side layout of one instance's re-sort bins: fills nbins, nbx, nby.
+void plan_bins(Instance& in, uint32_t cell_log2);
+
 // ---- launchers (all asynchronous on `stream`) ----------------------------
-void launch_agents(const StepGeom& g, const AgentSoA& a, void* trail, const DevParams* p,
-                   hipStream_t stream);
-void launch_trail(const StepGeom& g, const void* trail_in, void* trail_out, const DevParams* p,
-                  hipStream_t stream);
+void launch_agents(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t stream);
+void launch_trail(const Pool& pool, const void* trail_in, void* trail_out, hipStream_t stream);
 void launch_set_params(DevParams* dst, const DevParams& src, hipStream_t stream);
-void launch_init_agents(const AgentSoA& a, uint32_t n, uint64_t seed, uint32_t width,
-                        uint32_t height, float speed_min, float speed_max, hipStream_t stream);
-void launch_reassign_speeds(const AgentSoA& a, uint32_t n, uint64_t stream_seed,
+void launch_init_agents(const Instance& in, const AgentSoA& a, int pos64, uint64_t seed,
+                        float speed_min, float speed_max, hipStream_t stream);
+void launch_reassign_speeds(const Instance& in, const AgentSoA& a, uint64_t stream_seed,
                             float speed_min, float speed_max, hipStream_t stream);
-void launch_scale_positions(const AgentSoA& a, uint32_t n, float sx, float sy,
+void launch_scale_positions(const Instance& in, const AgentSoA& a, int pos64, float sx, float sy,
                             hipStream_t stream);
-void launch_pack_agents(const AgentSoA& a, uint32_t n, float* aos, hipStream_t stream);
-void launch_unpack_agents(const AgentSoA& a, uint32_t n, const float* aos, hipStream_t stream);
+void launch_pack_agents(const Instance& in, const AgentSoA& a, int pos64, float* aos,
+                        hipStream_t stream);
+void launch_unpack_agents(const Instance& in, const AgentSoA& a, int pos64, const float* aos,
+                          hipStream_t stream);
 void launch_decode_trail(const void* trail, int storage, uint64_t cells, float* out,
                          hipStream_t stream);
 void launch_encode_trail(const float* in, int storage, uint64_t cells, void* trail,
                          hipStream_t stream);
-// Deterministic two-pass f64 sum; `partials` holds kMassBlocks doubles.
-constexpr int kMassBlocks = 1024;
-void launch_total_mass(const void* trail, int storage, uint64_t cells, double* partials,
-                       double* out, hipStream_t stream);
+// Deterministic two-pass f64 sum per instance; `partials` holds
+// count * kMassBlocks doubles and `out` count doubles.
+constexpr int kMassBlocks = 256;
+void launch_total_mass(const Pool& pool, const void* trail, double* partials, double* out,
+                       hipStream_t stream);
 
 // ---- Morton re-sort ------------------------------------------------------
 struct SortScratch {
-  uint32_t* key;      // per agent: bin (Morton code)
-  uint32_t* rank;     // per agent: rank inside its bin
-  uint32_t* count;    // per bin
-  uint32_t* offset;   // per bin, exclusive prefix sum
-  uint32_t* block_sums;
-  uint32_t nbins;     // power of four
-  uint32_t cell_log2; // bin edge = 2^cell_log2 cells
+  uint32_t* key;       // per slot: global bin
+  uint32_t* rank;      // per slot: rank inside its bin
+  uint32_t* count;     // per bin
+  uint32_t* offset;    // per bin, exclusive prefix sum
+  uint32_t* level[3];  // tile sums of the multi-level scan
+  uint32_t level_len[3];
+  int levels;
 };
-uint32_t sort_bins_for(uint32_t width, uint32_t height, uint32_t cell_log2);
-uint32_t sort_block_sums_for(uint32_t nbins);
-void launch_morton_sort(const StepGeom& g, const AgentSoA& src, const AgentSoA& dst,
+// Sizes the scan levels for `nbins` bins; returns the number of u32 the
+// levels need in total (they are carved out of one allocation).
+uint32_t plan_scan(SortScratch& s, uint32_t nbins);
+void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& dst,
                         const SortScratch& s, hipStream_t stream);
-
-inline uint64_t storage_bytes(int storage) { return storage == SLM_STORAGE_F32 ? 4 : 2; }
 
 }  // namespace slm

@@ -3,11 +3,13 @@
 //
 // Agents are bucketed by the Morton (Z-order) code of the 2^k x 2^k cell bin
 // they stand in, then scattered bin by bin into the other agent buffer, so a
-// wave's lanes sense and deposit in nearby cells and share L2 lines.  The slot
-// -> original index table id[] moves with them.  The order inside a bin comes
-// from atomics and is not fixed, which is harmless: an agent's update depends
-// only on its own state, its original index and the step-start trail, and the
-// deposit marks commute, so any permutation gives the same result.
+// wave's lanes sense and deposit in nearby cells and share L2 lines.  Each
+// instance of a pool owns a contiguous range of bins, so agents stay inside
+// their instance's slot range.  The slot -> original index table id[] moves
+// with them.  The order inside a bin comes from atomics and is not fixed,
+// which is harmless: an agent's update depends only on its own state, its
+// original index and the step-start trail, and the deposit marks commute, so
+// any permutation gives the same result.
 #include "slm_internal.h"
 
 namespace slm {
@@ -15,7 +17,7 @@ namespace {
 
 constexpr int kScanBlock = 256;
 constexpr int kScanPerThread = 4;
-constexpr int kScanTile = kScanBlock * kScanPerThread;  // bins per scan block
+constexpr uint32_t kScanTile = kScanBlock * kScanPerThread;  // entries per scan block
 
 __device__ inline uint32_t spread_bits(uint32_t v) {  // 16 bits -> even positions
   v &= 0xffffu;
@@ -26,17 +28,12 @@ __device__ inline uint32_t spread_bits(uint32_t v) {  // 16 bits -> even positio
   return v;
 }
 
-struct BinGeom {
-  uint32_t cell_log2;
-  uint32_t nbx, nby;  // bins across / down
-};
-
-__device__ inline uint32_t bin_of(float x, float y, const BinGeom& b) {
-  uint32_t bx = static_cast<uint32_t>(x) >> b.cell_log2;  // x in [0, width]
-  uint32_t by = static_cast<uint32_t>(y) >> b.cell_log2;
-  bx = bx < b.nbx ? bx : b.nbx - 1u;
-  by = by < b.nby ? by : b.nby - 1u;
-  return spread_bits(bx) | (spread_bits(by) << 1);
+__device__ inline uint32_t bin_of(float x, float y, const Instance& in, uint32_t cell_log2) {
+  uint32_t bx = static_cast<uint32_t>(x) >> cell_log2;  // x in [0, width]
+  uint32_t by = static_cast<uint32_t>(y) >> cell_log2;
+  bx = bx < in.nbx ? bx : in.nbx - 1u;
+  by = by < in.nby ? by : in.nby - 1u;
+  return in.bin_off + (spread_bits(bx) | (spread_bits(by) << 1));
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
@@ -47,11 +44,24 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
 // Bins each agent and takes its rank inside the bin.  Lanes of a wave that share
 // a bin are served by one atomic (agents re-sorted earlier arrive grouped by
 // bin, so a wave usually holds one to four bins); ranks follow lane order.
-__global__ void count_kernel(AgentSoA a, uint32_t n, BinGeom b, uint32_t* key, uint32_t* rank,
-                             uint32_t* count) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = j < n;
-  uint32_t k = active ? bin_of(a.x[j], a.y[j], b) : 0u;
+__global__ void count_kernel(AgentSoA a, const Instance* insts, const uint32_t* block_inst,
+                             uint32_t ninst, uint32_t total, int pos64, uint32_t cell_log2,
+                             uint32_t* key, uint32_t* rank, uint32_t* count) {
+  uint32_t first = blockIdx.x * blockDim.x;
+  uint32_t last = min(first + blockDim.x, total) - 1u;
+  uint32_t j = first + threadIdx.x;
+  bool active = j < total;
+  uint32_t k = 0;
+  for (uint32_t ii = block_inst[blockIdx.x];; ii++) {  // block-uniform walk
+    const Instance in = insts[ii];
+    uint32_t end = in.agent_off + in.agent_count;
+    if (active && j >= in.agent_off && j < end) {
+      float x = pos64 ? static_cast<float>(a.x64[j]) : a.x[j];
+      float y = pos64 ? static_cast<float>(a.y64[j]) : a.y[j];
+      k = bin_of(x, y, in, cell_log2);
+    }
+    if (last < end || ii + 1u >= ninst) break;
+  }
   uint32_t lane = threadIdx.x & 63u;
   uint64_t below = (uint64_t(1) << lane) - 1u;
   uint64_t pending = __ballot(active);
@@ -72,7 +82,8 @@ __global__ void count_kernel(AgentSoA a, uint32_t n, BinGeom b, uint32_t* key, u
   }
 }
 
-// Exclusive scan of one tile of counts; writes the tile total to sums[tile].
+// Exclusive scan of one tile of `in` into `out` (in place allowed); writes
+// the tile total to sums[tile].
 __global__ __launch_bounds__(kScanBlock) void scan_tile_kernel(const uint32_t* in, uint32_t n,
                                                                uint32_t* out, uint32_t* sums) {
   __shared__ uint32_t sh[kScanBlock];
@@ -101,36 +112,31 @@ __global__ __launch_bounds__(kScanBlock) void scan_tile_kernel(const uint32_t* i
   if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = sh[threadIdx.x];
 }
 
-// Adds the exclusive prefix of the tile totals (scanned by one block) to every
-// tile; `ntiles` <= kScanTile.
+// out[i] += prefix[tile of i] (the exclusively scanned tile totals).
 __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(uint32_t* out, uint32_t n,
-                                                              const uint32_t* sums,
-                                                              uint32_t ntiles) {
-  __shared__ uint32_t prefix;
-  if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (uint32_t t = 0; t < blockIdx.x && t < ntiles; t++) s += sums[t];
-    prefix = s;
-  }
-  __syncthreads();
+                                                              const uint32_t* prefix) {
+  uint32_t add = prefix[blockIdx.x];
   uint32_t base = blockIdx.x * kScanTile;
   for (uint32_t k = threadIdx.x; k < kScanTile; k += kScanBlock)
-    if (base + k < n) out[base + k] += prefix;
+    if (base + k < n) out[base + k] += add;
 }
 
-__global__ void scatter_kernel(AgentSoA src, AgentSoA dst, uint32_t n, const uint32_t* key,
-                               const uint32_t* rank, const uint32_t* offset) {
+__global__ void scatter_kernel(AgentSoA src, AgentSoA dst, uint32_t n, int pos64,
+                               const uint32_t* key, const uint32_t* rank, const uint32_t* offset) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   uint32_t d = offset[key[j]] + rank[j];
-  dst.x[d] = src.x[j];
-  dst.y[d] = src.y[j];
+  if (pos64) {
+    dst.x64[d] = src.x64[j];
+    dst.y64[d] = src.y64[j];
+  } else {
+    dst.x[d] = src.x[j];
+    dst.y[d] = src.y[j];
+  }
   dst.th[d] = src.th[j];
   dst.v[d] = src.v[j];
   dst.id[d] = src.id[j];
 }
-
-inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
 
 inline uint32_t next_pow2(uint32_t v) {
   uint32_t p = 1;
@@ -138,30 +144,50 @@ inline uint32_t next_pow2(uint32_t v) {
   return p;
 }
 
-}  // namespace
-
-uint32_t sort_bins_for(uint32_t width, uint32_t height, uint32_t cell_log2) {
-  uint32_t side = 1u << cell_log2;
-  uint32_t nbx = (width + side - 1) / side, nby = (height + side - 1) / side;
-  uint32_t p = next_pow2(nbx > nby ? nbx : nby);
-  return p * p;
+// Exclusive scan of in[0..n) into out, using the levels from `level` on.
+void scan(const SortScratch& s, const uint32_t* in, uint32_t n, uint32_t* out, int level,
+          hipStream_t stream) {
+  uint32_t tiles = blocks_for(n, kScanTile);
+  scan_tile_kernel<<<tiles, kScanBlock, 0, stream>>>(in, n, out, s.level[level]);
+  if (tiles > 1) {
+    scan(s, s.level[level], tiles, s.level[level], level + 1, stream);
+    scan_add_kernel<<<tiles, kScanBlock, 0, stream>>>(out, n, s.level[level]);
+  }
 }
 
-uint32_t sort_block_sums_for(uint32_t nbins) { return blocks_for(nbins, kScanTile); }
+}  // namespace
 
-void launch_morton_sort(const StepGeom& g, const AgentSoA& src, const AgentSoA& dst,
+void plan_bins(Instance& in, uint32_t cell_log2) {
+  uint32_t side = 1u << cell_log2;
+  in.nbx = (in.width + side - 1) / side;
+  in.nby = (in.height + side - 1) / side;
+  uint32_t p = next_pow2(in.nbx > in.nby ? in.nbx : in.nby);
+  in.nbins = p * p;
+}
+
+uint32_t plan_scan(SortScratch& s, uint32_t nbins) {
+  uint32_t n = nbins, total = 0;
+  s.levels = 0;
+  for (int k = 0; k < 3; k++) {
+    n = blocks_for(n, kScanTile);
+    s.level_len[k] = n;
+    total += n;
+    s.levels = k + 1;
+    if (n <= 1) break;
+  }
+  return total;
+}
+
+void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& dst,
                         const SortScratch& s, hipStream_t stream) {
-  if (g.agents == 0) return;
-  uint32_t side = 1u << s.cell_log2;
-  BinGeom b{s.cell_log2, (g.width + side - 1) / side, (g.height + side - 1) / side};
-  uint32_t ntiles = sort_block_sums_for(s.nbins);
-  zero_u32_kernel<<<blocks_for(s.nbins, 256), 256, 0, stream>>>(s.count, s.nbins);
-  count_kernel<<<blocks_for(g.agents, 256), 256, 0, stream>>>(src, g.agents, b, s.key, s.rank,
-                                                             s.count);
-  scan_tile_kernel<<<ntiles, kScanBlock, 0, stream>>>(s.count, s.nbins, s.offset, s.block_sums);
-  scan_add_kernel<<<ntiles, kScanBlock, 0, stream>>>(s.offset, s.nbins, s.block_sums, ntiles);
-  scatter_kernel<<<blocks_for(g.agents, 256), 256, 0, stream>>>(src, dst, g.agents, s.key, s.rank,
-                                                               s.offset);
+  if (pool.total_agents == 0) return;
+  zero_u32_kernel<<<blocks_for(pool.total_bins, 256), 256, 0, stream>>>(s.count, pool.total_bins);
+  count_kernel<<<blocks_for(pool.total_agents, kAgentBlock), kAgentBlock, 0, stream>>>(
+      src, pool.inst, pool.agent_block_inst, pool.count, pool.total_agents, pool.pos64,
+      pool.sort_cell_log2, s.key, s.rank, s.count);
+  scan(s, s.count, pool.total_bins, s.offset, 0, stream);
+  scatter_kernel<<<blocks_for(pool.total_agents, 256), 256, 0, stream>>>(
+      src, dst, pool.total_agents, pool.pos64, s.key, s.rank, s.offset);
 }
 
 }  // namespace slm

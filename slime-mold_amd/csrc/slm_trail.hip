@@ -11,18 +11,16 @@
 // with the window summed in the shader's order (dy outer, dx inner, one
 // rounding per add) so the result is the reference's bit for bit.
 //
-// Strip kernel (row length a multiple of 16 bytes): each lane owns one 16-byte
-// column strip (4 f32 or 8 16-bit cells) and walks a tile of rows keeping the
-// rows above, at and below in registers; the left/right neighbour cells come
-// from the adjacent lanes, and only lanes at a wave edge or at the wrap load a
-// halo cell.  Rows of other lengths use a one-cell-per-lane kernel.
+// Work is cut into wave-sized units over a pool of instances.  Strip unit (row
+// length a multiple of 16 bytes): each lane owns one 16-byte column strip (4
+// f32 or 8 16-bit cells) and walks `tile_rows` rows keeping the rows above, at
+// and below in registers; the left/right neighbour cells come from the
+// adjacent lanes, and only lanes at a wave edge or at the wrap load a halo
+// cell.  Rows of other lengths use 64-cell units, one cell per lane.
 #include "slm_internal.h"
 
 namespace slm {
 namespace {
-
-constexpr int kTrailBlock = 256;
-constexpr int kWavesPerBlock = kTrailBlock / 64;
 
 __device__ inline uint32_t xcd_logical_block(uint32_t b, uint32_t nb) {
   uint32_t xcd = b & 7u, idx = b >> 3;
@@ -99,16 +97,13 @@ __device__ inline float deposit_decay(uint32_t raw, const DevParams& p) {
   return u > 0.0f ? u : 0.0f;
 }
 
-struct TrailArgs {
-  const void* in;
-  void* out;
-  const DevParams* p;
-  uint32_t w, h;
-  uint32_t tile_rows;  // rows per wave tile
-  uint32_t ntx;        // wave tiles across a row
-  uint32_t ntiles;     // wave tiles in the grid
-  int xcd_remap;
-};
+// mix(centre, window/9, rate) (compute.wgsl:193-194).
+__device__ inline float diffuse_out(float centre, float sum, const DevParams& p) {
+  float avg = sum / 9.0f;
+  float l = centre * p.one_minus_rate;
+  float r = avg * p.rate;
+  return l + r;
+}
 
 template <int S>
 struct RawRow {
@@ -139,28 +134,22 @@ __device__ inline void finish_row(float (&t)[Codec<S>::kVec + 2], const RawRow<S
   t[V + 1] = need_right ? deposit_decay<S>(r.right, p) : from_right;
 }
 
+// One strip unit: 64*V columns x tile_rows rows of instance `in`.
 template <int S>
-__global__ __launch_bounds__(kTrailBlock) void trail_strip_kernel(TrailArgs a) {
-  using C = Codec<S>;
-  using elem = typename C::elem;
-  constexpr int V = C::kVec;
-  const DevParams p = *a.p;
-  uint32_t blk = a.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  uint32_t tile = blk * kWavesPerBlock + wave;
-  if (tile >= a.ntiles) return;  // uniform per wave
-  uint32_t tx = tile % a.ntx, ty = tile / a.ntx;
-  const uint32_t w = a.w, h = a.h;
+__device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Codec<S>::elem* out,
+                                  const Instance& inst, const DevParams& p, uint32_t tile,
+                                  uint32_t lane) {
+  constexpr int V = Codec<S>::kVec;
+  const uint32_t w = inst.width, h = inst.height;
+  uint32_t tx = tile % inst.ntx, ty = tile / inst.ntx;
   uint32_t c = (tx * 64u + lane) * V;
   bool valid = c < w;
   bool need_left = valid && lane == 0u;
   bool need_right = valid && (lane == 63u || c + V >= w);
   uint32_t lcol = (c == 0u ? w : c) - 1u;
   uint32_t rcol = c + V >= w ? c + V - w : c + V;
-  uint32_t y0 = ty * a.tile_rows;
-  uint32_t y1 = min(y0 + a.tile_rows, h);
-  const elem* in = static_cast<const elem*>(a.in);
-  elem* out = static_cast<elem*>(a.out);
+  uint32_t y0 = ty * inst.tile_rows;
+  uint32_t y1 = min(y0 + inst.tile_rows, h);
   auto row_ptr = [&](uint32_t r) { return in + static_cast<size_t>(r) * w; };
   auto wrap_up = [&](uint32_t r) { return r == 0u ? h - 1u : r - 1u; };
   auto wrap_down = [&](uint32_t r) { return r + 1u >= h ? r + 1u - h : r + 1u; };
@@ -193,12 +182,9 @@ __global__ __launch_bounds__(kTrailBlock) void trail_strip_kernel(TrailArgs a) {
         s = s + next[k];
         s = s + next[k + 1];
         s = s + next[k + 2];
-        float avg = s / 9.0f;
-        float l = cur[k + 1] * p.one_minus_rate;
-        float r = avg * p.rate;
-        e[k] = C::encode(l + r);
+        e[k] = Codec<S>::encode(diffuse_out(cur[k + 1], s, p));
       }
-      *reinterpret_cast<uint4*>(out + static_cast<size_t>(y) * w + c) = C::pack(e);
+      *reinterpret_cast<uint4*>(out + static_cast<size_t>(y) * w + c) = Codec<S>::pack(e);
     }
 #pragma unroll
     for (int k = 0; k < V + 2; k++) {
@@ -210,17 +196,15 @@ __global__ __launch_bounds__(kTrailBlock) void trail_strip_kernel(TrailArgs a) {
   }
 }
 
-// One cell per lane, any row length.
+// One 64-cell unit, one cell per lane, any row length.
 template <int S>
-__global__ __launch_bounds__(kTrailBlock) void trail_cell_kernel(TrailArgs a) {
-  using C = Codec<S>;
-  using elem = typename C::elem;
-  const DevParams p = *a.p;
-  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kTrailBlock + threadIdx.x;
-  const uint32_t w = a.w, h = a.h;
+__device__ inline void cell_unit(const typename Codec<S>::elem* in, typename Codec<S>::elem* out,
+                                 const Instance& inst, const DevParams& p, uint32_t unit,
+                                 uint32_t lane) {
+  const uint32_t w = inst.width, h = inst.height;
+  uint64_t i = static_cast<uint64_t>(unit) * 64u + lane;
   if (i >= static_cast<uint64_t>(w) * h) return;
   uint32_t x = static_cast<uint32_t>(i % w), y = static_cast<uint32_t>(i / w);
-  const elem* in = static_cast<const elem*>(a.in);
   uint32_t xs[3] = {x == 0u ? w - 1u : x - 1u, x, x + 1u >= w ? x + 1u - w : x + 1u};
   uint32_t ys[3] = {y == 0u ? h - 1u : y - 1u, y, y + 1u >= h ? y + 1u - h : y + 1u};
   float s = 0.0f, centre = 0.0f;
@@ -233,10 +217,38 @@ __global__ __launch_bounds__(kTrailBlock) void trail_cell_kernel(TrailArgs a) {
       s = (dy == 0 && dx == 0) ? t : s + t;
     }
   }
-  float avg = s / 9.0f;
-  float l = centre * p.one_minus_rate;
-  float r = avg * p.rate;
-  static_cast<elem*>(a.out)[i] = static_cast<elem>(C::encode(l + r));
+  out[i] = static_cast<typename Codec<S>::elem>(Codec<S>::encode(diffuse_out(centre, s, p)));
+}
+
+struct TrailArgs {
+  const void* in;
+  void* out;
+  const Instance* inst;
+  const DevParams* params;
+  const uint32_t* block_inst;
+  uint32_t count, total_units;
+  int xcd_remap;
+};
+
+template <int S>
+__global__ __launch_bounds__(kTrailBlock) void trail_kernel(TrailArgs a) {
+  using elem = typename Codec<S>::elem;
+  uint32_t blk = a.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  uint32_t lane = threadIdx.x & 63u;
+  uint32_t u = blk * kTrailWavesPerBlock + wave;
+  if (u >= a.total_units) return;  // uniform per wave
+  uint32_t ii = a.block_inst[blk];
+  while (u >= a.inst[ii].unit_off + a.inst[ii].units) ii++;
+  const Instance inst = a.inst[ii];
+  const DevParams p = a.params[ii];
+  const elem* in = static_cast<const elem*>(a.in) + inst.trail_off;
+  elem* out = static_cast<elem*>(a.out) + inst.trail_off;
+  uint32_t unit = u - inst.unit_off;
+  if (inst.ntx > 0)
+    strip_unit<S>(in, out, inst, p, unit, lane);
+  else
+    cell_unit<S>(in, out, inst, p, unit, lane);
 }
 
 __global__ void set_params_kernel(DevParams* dst, DevParams src) { *dst = src; }
@@ -256,11 +268,15 @@ __global__ void encode_trail_kernel(const float* in, uint64_t cells, void* t) {
       static_cast<typename Codec<S>::elem>(Codec<S>::encode(in[i]));
 }
 
+// partials[inst * kMassBlocks + b]: f64 sum of a fixed stride of inst's cells.
 template <int S>
-__global__ __launch_bounds__(256) void mass_partial_kernel(const void* t, uint64_t cells,
+__global__ __launch_bounds__(256) void mass_partial_kernel(const void* t, const Instance* insts,
                                                            double* partials) {
   __shared__ double red[256];
-  const typename Codec<S>::elem* e = static_cast<const typename Codec<S>::elem*>(t);
+  const Instance in = insts[blockIdx.y];
+  const typename Codec<S>::elem* e =
+      static_cast<const typename Codec<S>::elem*>(t) + in.trail_off;
+  uint64_t cells = static_cast<uint64_t>(in.width) * in.height;
   double acc = 0.0;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < cells;
        i += static_cast<uint64_t>(gridDim.x) * 256)
@@ -271,48 +287,50 @@ __global__ __launch_bounds__(256) void mass_partial_kernel(const void* t, uint64
     if (threadIdx.x < static_cast<unsigned>(s)) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) partials[static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
 }
 
 __global__ __launch_bounds__(256) void mass_final_kernel(const double* partials, int n,
                                                          double* out) {
   __shared__ double red[256];
+  const double* p = partials + static_cast<size_t>(blockIdx.x) * n;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) acc += partials[i];
+  for (int i = threadIdx.x; i < n; i += 256) acc += p[i];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < static_cast<unsigned>(s)) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 template <int S>
-void launch_trail_t(const StepGeom& g, const void* in, void* out, const DevParams* p,
-                    hipStream_t stream) {
-  constexpr int V = Codec<S>::kVec;
-  TrailArgs a{in, out, p, g.width, g.height, 0, 0, 0, g.xcd_remap};
-  if (g.width % V == 0) {
-    a.ntx = (g.width + 64u * V - 1u) / (64u * V);
-    // Rows per wave tile: the longest of 64..4 that still gives >= 8192 waves.
-    uint32_t rows = 64;
-    while (rows > 4 && static_cast<uint64_t>(a.ntx) * ((g.height + rows - 1) / rows) < 8192u)
-      rows >>= 1;
-    a.tile_rows = rows;
-    a.ntiles = a.ntx * ((g.height + rows - 1u) / rows);
-    uint32_t nb = (a.ntiles + kWavesPerBlock - 1u) / kWavesPerBlock;
-    trail_strip_kernel<S><<<nb, kTrailBlock, 0, stream>>>(a);
-  } else {
-    uint64_t cells = static_cast<uint64_t>(g.width) * g.height;
-    uint32_t nb = static_cast<uint32_t>((cells + kTrailBlock - 1) / kTrailBlock);
-    trail_cell_kernel<S><<<nb, kTrailBlock, 0, stream>>>(a);
-  }
+void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t stream) {
+  TrailArgs a{in, out, pool.inst, pool.params, pool.unit_block_inst, pool.count,
+              pool.total_units, pool.xcd_remap};
+  uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
+  trail_kernel<S><<<nb, kTrailBlock, 0, stream>>>(a);
 }
 
-inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
-
 }  // namespace
+
+void plan_trail(Instance& in, int storage, uint32_t target_waves) {
+  uint32_t v = storage == SLM_STORAGE_F32 ? 4u : 8u;
+  if (in.width % v == 0) {
+    in.ntx = (in.width + 64u * v - 1u) / (64u * v);
+    // rows per strip unit: the longest of 64..4 that still gives target_waves units
+    uint32_t rows = 64;
+    while (rows > 4 && static_cast<uint64_t>(in.ntx) * ((in.height + rows - 1) / rows) < target_waves)
+      rows >>= 1;
+    in.tile_rows = rows;
+    in.units = in.ntx * ((in.height + rows - 1u) / rows);
+  } else {
+    in.ntx = 0;
+    in.tile_rows = 0;
+    in.units = blocks_for(static_cast<uint64_t>(in.width) * in.height, 64u);
+  }
+}
 
 DevParams make_dev_params(const slm_params& s) {
   DevParams d;
@@ -332,17 +350,17 @@ DevParams make_dev_params(const slm_params& s) {
   return d;
 }
 
-void launch_trail(const StepGeom& g, const void* in, void* out, const DevParams* p,
-                  hipStream_t stream) {
-  switch (g.storage) {
+void launch_trail(const Pool& pool, const void* in, void* out, hipStream_t stream) {
+  if (pool.total_units == 0) return;
+  switch (pool.storage) {
     case SLM_STORAGE_BF16:
-      launch_trail_t<SLM_STORAGE_BF16>(g, in, out, p, stream);
+      launch_trail_t<SLM_STORAGE_BF16>(pool, in, out, stream);
       break;
     case SLM_STORAGE_F16:
-      launch_trail_t<SLM_STORAGE_F16>(g, in, out, p, stream);
+      launch_trail_t<SLM_STORAGE_F16>(pool, in, out, stream);
       break;
     default:
-      launch_trail_t<SLM_STORAGE_F32>(g, in, out, p, stream);
+      launch_trail_t<SLM_STORAGE_F32>(pool, in, out, stream);
       break;
   }
 }
@@ -375,15 +393,17 @@ void launch_encode_trail(const float* in, int storage, uint64_t cells, void* t,
     encode_trail_kernel<SLM_STORAGE_F32><<<nb, 256, 0, stream>>>(in, cells, t);
 }
 
-void launch_total_mass(const void* t, int storage, uint64_t cells, double* partials, double* out,
+void launch_total_mass(const Pool& pool, const void* t, double* partials, double* out,
                        hipStream_t stream) {
-  if (storage == SLM_STORAGE_BF16)
-    mass_partial_kernel<SLM_STORAGE_BF16><<<kMassBlocks, 256, 0, stream>>>(t, cells, partials);
-  else if (storage == SLM_STORAGE_F16)
-    mass_partial_kernel<SLM_STORAGE_F16><<<kMassBlocks, 256, 0, stream>>>(t, cells, partials);
+  if (pool.count == 0) return;
+  dim3 grid(kMassBlocks, pool.count);
+  if (pool.storage == SLM_STORAGE_BF16)
+    mass_partial_kernel<SLM_STORAGE_BF16><<<grid, 256, 0, stream>>>(t, pool.inst, partials);
+  else if (pool.storage == SLM_STORAGE_F16)
+    mass_partial_kernel<SLM_STORAGE_F16><<<grid, 256, 0, stream>>>(t, pool.inst, partials);
   else
-    mass_partial_kernel<SLM_STORAGE_F32><<<kMassBlocks, 256, 0, stream>>>(t, cells, partials);
-  mass_final_kernel<<<1, 256, 0, stream>>>(partials, kMassBlocks, out);
+    mass_partial_kernel<SLM_STORAGE_F32><<<grid, 256, 0, stream>>>(t, pool.inst, partials);
+  mass_final_kernel<<<pool.count, 256, 0, stream>>>(partials, kMassBlocks, out);
 }
 
 }  // namespace slm

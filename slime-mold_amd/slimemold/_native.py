@@ -24,6 +24,7 @@ STORAGE_F16 = 2
 STORAGE_NAMES = {"f32": STORAGE_F32, "bf16": STORAGE_BF16, "f16": STORAGE_F16}
 
 FLAG_NO_XCD_REMAP = 0x1
+FLAG_F64_POSITIONS = 0x2
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -99,6 +100,39 @@ class Timing(ctypes.Structure):
     ]
 
 
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("agent_count", ctypes.c_uint64),
+        ("seed", ctypes.c_uint64),
+        ("params", Params),
+    ]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("storage", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("resort_interval", ctypes.c_uint32),
+        ("graph_chunk", ctypes.c_uint32),
+        ("sort_cell_log2", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 6),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint32),
+        ("storage", ctypes.c_int32),
+        ("total_agents", ctypes.c_uint64),
+        ("total_cells", ctypes.c_uint64),
+        ("steps_done", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64),
+    ]
+
+
 _P = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
 
@@ -124,6 +158,21 @@ SIGNATURES = {
     "slm_write_trail": [_P, _FP, ctypes.c_uint64],
     "slm_total_mass": [_P, ctypes.POINTER(ctypes.c_double)],
     "slm_get_info": [_P, ctypes.POINTER(Info)],
+    "slm_batch_config_default": [ctypes.POINTER(BatchConfig)],
+    "slm_batch_create": [ctypes.POINTER(BatchConfig), ctypes.POINTER(InstanceDesc), ctypes.c_uint32,
+                         ctypes.POINTER(_P)],
+    "slm_batch_destroy": [_P],
+    "slm_batch_step": [_P, ctypes.c_uint32],
+    "slm_batch_step_timed": [_P, ctypes.c_uint32, ctypes.POINTER(Timing)],
+    "slm_batch_synchronize": [_P],
+    "slm_batch_set_params": [_P, ctypes.c_uint32, ctypes.POINTER(Params)],
+    "slm_batch_clear": [_P, ctypes.c_uint32],
+    "slm_batch_read_agents": [_P, ctypes.c_uint32, _FP, ctypes.c_uint64],
+    "slm_batch_write_agents": [_P, ctypes.c_uint32, _FP, ctypes.c_uint64],
+    "slm_batch_read_trail": [_P, ctypes.c_uint32, _FP, ctypes.c_uint64],
+    "slm_batch_write_trail": [_P, ctypes.c_uint32, _FP, ctypes.c_uint64],
+    "slm_batch_total_mass": [_P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32],
+    "slm_batch_get_info": [_P, ctypes.POINTER(BatchInfo)],
 }
 
 _lib = None
@@ -165,6 +214,12 @@ def default_params() -> Params:
     p = Params()
     check(lib().slm_params_default(ctypes.byref(p)))
     return p
+
+
+def default_batch_config() -> BatchConfig:
+    c = BatchConfig()
+    check(lib().slm_batch_config_default(ctypes.byref(c)))
+    return c
 
 
 def default_config() -> Config:

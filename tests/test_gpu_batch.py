@@ -1,0 +1,126 @@
+"""Batches and the f64-position mode on the GPU, bit for bit.
+
+A batch instance must equal the same simulation run alone (and the oracle);
+the f64-position mode must equal the oracle's f64-position restatement.
+"""
+import numpy as np
+import pytest
+
+from parity_util import PRESET_NAMES, assert_bitwise, preset
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_specs(slm, seed=0):
+    """Uneven instances: powers of two, ragged widths (one-cell-per-lane path),
+    tiny grids, an empty instance and every preset."""
+    rng = np.random.default_rng(seed)
+    sizes = [(256, 256), (512, 128), (37, 23), (1030, 40), (64, 64), (1, 1), (260, 2),
+             (128, 512), (96, 80), (1032, 50)]
+    specs = []
+    for k, (w, h) in enumerate(sizes):
+        n = int(min(w * h * 2, rng.integers(0, 20000)))
+        if k == 4:
+            n = 0
+        specs.append(slm.InstanceSpec(w, h, n, seed=100 + k,
+                                      params=preset(slm, PRESET_NAMES[k % len(PRESET_NAMES)])))
+    return specs
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_batch_instances_equal_single_runs_and_oracle(slm, orc, storage):
+    specs = mixed_specs(slm)
+    steps = 12
+    print(f"[batch] {len(specs)} instances storage={storage} steps={steps}", flush=True)
+    with slm.Batch(specs, storage=storage, graph_chunk=5) as b:
+        starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+        b.step(steps)
+        got = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+        masses = b.total_mass()
+    for k, s in enumerate(specs):
+        with slm.Simulation(s.width, s.height, s.agent_count, seed=s.seed, params=s.params,
+                            storage=storage, graph_chunk=0) as sim:
+            assert_bitwise(sim.read_agents(), starts[k][0], f"instance {k} init")
+            sim.step(steps)
+            assert_bitwise(got[k][0], sim.read_agents(), f"instance {k} agents vs single")
+            assert_bitwise(got[k][1], sim.read_trail(), f"instance {k} trail vs single")
+        a, t = starts[k]
+        orc.step(a, t, s.params, steps, storage=storage)
+        assert_bitwise(got[k][0], a, f"instance {k} agents vs oracle")
+        assert_bitwise(got[k][1], t, f"instance {k} trail vs oracle")
+        assert masses[k] == pytest.approx(got[k][1].astype(np.float64).sum(), rel=1e-12, abs=1e-9)
+
+
+def test_batch_resort_and_graphs_do_not_change_results(slm):
+    specs = mixed_specs(slm, seed=1)
+    outs = []
+    for chunk, resort, cl in [(0, 0, 4), (10, 3, 4), (7, 2, 2)]:
+        with slm.Batch(specs, graph_chunk=chunk, resort_interval=resort, sort_cell_log2=cl) as b:
+            b.step(21)
+            outs.append([(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))])
+    for other in outs[1:]:
+        for k in range(len(specs)):
+            assert_bitwise(other[k][0], outs[0][k][0], f"instance {k} agents")
+            assert_bitwise(other[k][1], outs[0][k][1], f"instance {k} trail")
+
+
+def test_batch_per_instance_params_and_clear(slm, orc):
+    specs = [slm.InstanceSpec(128, 128, 3000, seed=s, params=preset(slm, "Default"))
+             for s in range(3)]
+    with slm.Batch(specs, graph_chunk=4) as b:
+        states = [(b.read_agents(k), b.read_trail(k)) for k in range(3)]
+        b.set_params(1, preset(slm, "Mesh"))
+        b.step(8)
+        b.clear(2)
+        b.step(3)
+        for k, nm in enumerate(["Default", "Mesh", "Default"]):
+            a, t = states[k]
+            orc.step(a, t, preset(slm, nm), 8)
+            if k == 2:
+                t[:] = 0.0
+            orc.step(a, t, preset(slm, nm), 3)
+            assert_bitwise(b.read_agents(k), a, f"instance {k} agents")
+            assert_bitwise(b.read_trail(k), t, f"instance {k} trail")
+
+
+def test_batch_rejects_bad_instances(slm):
+    with pytest.raises(slm.SlimeMoldError):
+        slm.Batch([slm.InstanceSpec(0, 10, 5)])
+    with pytest.raises(slm.SlimeMoldError):
+        slm.Batch([])
+    with slm.Batch([slm.InstanceSpec(16, 16, 5)]) as b:
+        import ctypes
+        buf = np.zeros(256, np.float32)
+        ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        assert slm.lib().slm_batch_read_trail(b.handle, 1, ptr, 256) == -1
+        assert "out of range" in slm.lib().slm_last_error().decode()
+        assert slm.lib().slm_batch_read_trail(b.handle, 0, ptr, 255) == -1
+
+
+@pytest.mark.parametrize("name", ["Default", "Waves", "Snake"])
+def test_f64_positions_match_oracle(slm, orc, name):
+    w, h, n, steps = 700, 500, 1 << 14, 16
+    s = preset(slm, name)
+    with slm.Simulation(w, h, n, seed=8, params=s, f64_positions=True, graph_chunk=4,
+                        resort_interval=5) as sim:
+        a0 = sim.read_agents()
+        t = sim.read_trail()
+        sim.step(steps)
+        ga, gt = sim.read_agents(), sim.read_trail()
+    a = a0.astype(np.float64)
+    orc.step_p64(a, t, s, steps)
+    assert_bitwise(ga, a.astype(np.float32), "f64-position agents (rounded to f32)")
+    assert_bitwise(gt, t, "trail")
+
+
+def test_f64_positions_in_a_batch(slm):
+    specs = mixed_specs(slm, seed=2)[:6]
+    with slm.Batch(specs, f64_positions=True, storage="f16") as b:
+        b.step(9)
+        got = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+    for k, sp in enumerate(specs):
+        with slm.Simulation(sp.width, sp.height, sp.agent_count, seed=sp.seed, params=sp.params,
+                            storage="f16", f64_positions=True) as sim:
+            sim.step(9)
+            assert_bitwise(got[k][0], sim.read_agents(), f"instance {k} agents")
+            assert_bitwise(got[k][1], sim.read_trail(), f"instance {k} trail")
