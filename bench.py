@@ -1,16 +1,20 @@
 """Throughput of the slime-mold step on one MI355X.
 
-Default workload (the plain-FP32-at-scale case): 4096x4096 f32 trail, 2^24
-agents initialised from seed 1, the presets Default -> Sponge -> Firecracker
-Trees run back to back (the K timed steps split in three), steps enqueued as
-captured hipGraphs of 100 steps.  A step is one agent pass plus one fused
-deposit + decay + diffuse pass over the whole grid.
+Plain leg (`value`): 4096x4096 f32 trail, 2^24 agents initialised from seed 1,
+the presets Default -> Sponge -> Firecracker Trees run back to back (the K timed
+steps split in three), steps enqueued as captured hipGraphs of 100 steps.  A
+step is one agent pass plus one fused deposit + decay + diffuse pass over the
+whole grid.
 
-Prints one JSON line: value = agent-updates/s over the timed region, plus the
-roofline of the dominant kernel (per-kernel device time from HIP events on the
-library's stream, eager launches right after the timed region), the diffuse
-kernel's achieved bandwidth, and the CPU oracle timed on a bounded sample of
-the same state on the host cores.
+Batched leg: the 256-instance mix (grids 256^2..2048^2, 2^14..2^22 agents,
+parameters swept over the preset range) with a bf16 trail, all instances
+advanced by one agent launch and one trail launch per step.
+
+Prints one JSON line: value = plain agent-updates/s over the timed region, the
+roofline of the plain leg's dominant kernel (per-kernel device time from HIP
+events on the library's stream, eager launches right after the timed region),
+the diffuse kernel's achieved bandwidth, the batched leg, and the CPU oracle
+timed on a bounded sample of the plain leg's state on the host cores.
 """
 import argparse
 import json
@@ -25,17 +29,22 @@ for _p in (ROOT, os.path.join(ROOT, "slime-mold_amd")):
 
 METRIC = "agent-updates/s + diffuse GB/s (% of HBM peak), plain FP32 vs batched BF16"
 HBM_PEAK_GBPS = 8000.0
-PRESETS = ["Default", "Sponge", "Firecracker Trees"]
+STORAGE_BYTES = {"f32": 4, "bf16": 2, "f16": 2}
+
+# Counter calibration measured with --copy-probe-mib 1024 under rocprofv3 on
+# MI355X: FETCH_SIZE reads half the bytes a plain copy moves, WRITE_SIZE all.
+FETCH_SIZE_SCALE = 2.0
+WRITE_SIZE_SCALE = 1.0
 
 
-def agent_bytes(storage_bytes):
+def agent_bytes(sb, pos64=False):
     # 16 B state read + 16 B state write + 12 sensor taps + deposit read and write
-    return 32 + 14 * storage_bytes
+    return 32 + 14 * sb + (16 if pos64 else 0)
 
 
-def trail_bytes(storage_bytes):
+def trail_bytes(sb):
     # one read and one write of every cell
-    return 2 * storage_bytes
+    return 2 * sb
 
 
 def parse():
@@ -43,6 +52,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="plain", choices=["plain", "combined"],
+                    help="plain: plain + batched legs; combined: 1024-instance mix + "
+                         "16384^2 / 2^28-agent instance, bf16")
     ap.add_argument("--width", type=int, default=4096)
     ap.add_argument("--height", type=int, default=4096)
     ap.add_argument("--agents-log2", type=int, default=24)
@@ -53,10 +65,35 @@ def parse():
     ap.add_argument("--sort-cell-log2", type=int, default=4, help="re-sort bin edge 2^k cells")
     ap.add_argument("--timing-steps", type=int, default=30,
                     help="eager steps per preset for the per-kernel event timing")
+    ap.add_argument("--batch-steps", type=int, default=1000, help="batched leg steps (0: skip)")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-storage", default="bf16", choices=["f32", "bf16", "f16"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only copy a buffer of this many MiB device to device three times "
+                         "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes)")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size for counter calibration."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
+                      "copies": 3}))
+    return 0
 
 
 def cpu_baseline(orc, agents, trail, settings, seconds, threads):
@@ -73,133 +110,198 @@ def cpu_baseline(orc, agents, trail, settings, seconds, threads):
     return n, dt
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        print("bench.py: only --gpus 1 is supported (single-GPU build)", file=sys.stderr)
-        return 2
-    import slimemold
+def kernel_times(handle, presets, steps, set_params):
+    """Average device time per launch of each kernel over eager steps."""
+    tot = {"agents_ms": 0.0, "trail_ms": 0.0, "sort_ms": 0.0, "agents": 0, "trail": 0, "sort": 0}
+    for p in presets:
+        if set_params is not None:
+            set_params(p)
+        t = handle.step_timed(steps)
+        tot["agents_ms"] += t.agents_ms
+        tot["trail_ms"] += t.trail_ms
+        tot["sort_ms"] += t.sort_ms
+        tot["agents"] += t.agents_launches
+        tot["trail"] += t.trail_launches
+        tot["sort"] += t.sort_launches
+    return (tot["agents_ms"] / max(1, tot["agents"]) / 1e3,
+            tot["trail_ms"] / max(1, tot["trail"]) / 1e3,
+            tot["sort_ms"] / tot["sort"] / 1e3 if tot["sort"] else 0.0)
 
-    if slimemold.device_count() < 1:
-        print("bench.py: no HIP device", file=sys.stderr)
-        return 1
-    pm = slimemold.init_preset_manager()
-    presets = [pm.get_preset(n).settings for n in PRESETS]
+
+def roofline(name, bytes_per_launch, seconds):
+    gbps = bytes_per_launch / seconds / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4), "traffic": None,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(seconds * 1e6, 2)}
+
+
+def plain_leg(slm, args):
+    presets = [slm.workloads.preset(n) for n in slm.workloads.PLAIN_PRESETS]
     W, H, N = args.width, args.height, 1 << args.agents_log2
-    sb = 4 if args.storage == "f32" else 2
+    sb = STORAGE_BYTES[args.storage]
     K = args.steps
     seg = [K // 3, K // 3, K - 2 * (K // 3)]
-
-    sim = slimemold.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
-                               resort_interval=args.resort, graph_chunk=args.graph_chunk,
-                               sort_cell_log2=args.sort_cell_log2)
-    print(f"[bench] {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
-          f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}", file=sys.stderr,
-          flush=True)
-    # snapshot of the initial state for the CPU sample
-    agents0 = sim.read_agents() if args.cpu_seconds > 0 else None
-    trail0 = sim.read_trail() if args.cpu_seconds > 0 else None
-
+    sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
+                         resort_interval=args.resort, graph_chunk=args.graph_chunk,
+                         sort_cell_log2=args.sort_cell_log2)
+    log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
+        f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}")
+    state0 = (sim.read_agents(), sim.read_trail()) if args.cpu_seconds > 0 else None
     sim.step(args.warmup)
     sim.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-
     t0 = time.perf_counter()
     for p, n in zip(presets, seg):
         sim.set_params(p)
         sim.step(n)
     sim.synchronize()
     elapsed = time.perf_counter() - t0
-    print(f"[bench] timed {K} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
-
-    # per-kernel device time, eager launches with events on the library's stream
-    tim = {"agents_ms": 0.0, "trail_ms": 0.0, "sort_ms": 0.0, "agents": 0, "trail": 0, "sort": 0}
-    for p in presets:
-        sim.set_params(p)
-        t = sim.step_timed(args.timing_steps)
-        tim["agents_ms"] += t.agents_ms
-        tim["trail_ms"] += t.trail_ms
-        tim["sort_ms"] += t.sort_ms
-        tim["agents"] += t.agents_launches
-        tim["trail"] += t.trail_launches
-        tim["sort"] += t.sort_launches
-    agents_avg_s = tim["agents_ms"] / max(1, tim["agents"]) / 1e3
-    trail_avg_s = tim["trail_ms"] / max(1, tim["trail"]) / 1e3
-    sort_avg_s = tim["sort_ms"] / max(1, tim["sort"]) / 1e3 if tim["sort"] else 0.0
+    log(f"plain: timed {K} steps in {elapsed:.3f} s")
+    a_s, t_s, s_s = kernel_times(sim, presets, args.timing_steps, sim.set_params)
     mass = sim.total_mass()
     sim.close()
-
-    a_bytes = agent_bytes(sb) * N
-    t_bytes = trail_bytes(sb) * W * H
-    a_gbps = a_bytes / agents_avg_s / 1e9
-    t_gbps = t_bytes / trail_avg_s / 1e9
-    dominant = "agents" if agents_avg_s >= trail_avg_s else "trail"
-    roof = {
-        "kernel": "agents_kernel" if dominant == "agents" else "trail_strip_kernel",
-        "bound": "hbm",
-        "achieved": round(a_gbps if dominant == "agents" else t_gbps, 1),
-        "peak": HBM_PEAK_GBPS,
-        "unit": "GB/s",
-        "frac": round((a_gbps if dominant == "agents" else t_gbps) / HBM_PEAK_GBPS, 4),
-        "traffic": None,
-        "bytes_per_launch": a_bytes if dominant == "agents" else t_bytes,
-        "avg_launch_us": round((agents_avg_s if dominant == "agents" else trail_avg_s) * 1e6, 2),
+    a_bytes, t_bytes = agent_bytes(sb) * N, trail_bytes(sb) * W * H
+    return {
+        "value": N * K / elapsed, "elapsed": elapsed, "steps": K, "segments": seg,
+        "agents_s": a_s, "trail_s": t_s, "sort_s": s_s, "mass": mass,
+        "agents_roof": roofline("agents_kernel", a_bytes, a_s),
+        "trail_roof": roofline("trail_kernel", t_bytes, t_s),
+        "cells": W * H, "agents": N, "state0": state0, "presets": presets,
     }
+
+
+def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label):
+    sb = STORAGE_BYTES[storage]
+    log(f"{label}: {len(specs)} instances storage={storage} creating")
+    b = slm.Batch(specs, storage=storage, resort_interval=resort, graph_chunk=graph_chunk,
+                  sort_cell_log2=cell_log2)
+    info = b.info()
+    log(f"{label}: {info.total_agents} agents, {info.total_cells} cells, "
+        f"{info.device_bytes / 2**30:.1f} GiB; warmup {warmup}")
+    b.step(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"{label}: timed {steps} steps in {elapsed:.3f} s")
+    a_s, t_s, s_s = kernel_times(b, [None], 20, None)
+    masses = b.total_mass()
+    b.close()
+    n_agents, n_cells = info.total_agents, info.total_cells
+    return {
+        "instances": len(specs), "storage": storage, "steps": steps, "warmup": warmup,
+        "total_agents": n_agents, "total_cells": n_cells,
+        "agent_updates_per_s": n_agents * steps / elapsed,
+        "cell_updates_per_s": n_cells * steps / elapsed,
+        "ms_per_step": elapsed / steps * 1e3,
+        "agents_kernel_us": round(a_s * 1e6, 2),
+        "trail_kernel_us": round(t_s * 1e6, 2),
+        "sort_us": round(s_s * 1e6, 2),
+        "diffuse_gbps": round(trail_bytes(sb) * n_cells / t_s / 1e9, 1),
+        "diffuse_pct_hbm_peak": round(100 * trail_bytes(sb) * n_cells / t_s / 1e9 / HBM_PEAK_GBPS, 2),
+        "agents_gbps": round(agent_bytes(sb) * n_agents / a_s / 1e9, 1),
+        "resort_interval": resort, "graph_chunk": graph_chunk, "sort_cell_log2": cell_log2,
+        "total_mass_sum": float(masses.sum()),
+    }
+
+
+def main():
+    args = parse()
+    if args.copy_probe_mib:
+        return copy_probe(args.copy_probe_mib)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        print("bench.py: only --gpus 1 is supported (single-GPU build)", file=sys.stderr)
+        return 2
+    import slimemold as slm
+    import slimemold.workloads  # noqa: F401
+
+    if slm.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+
+    if args.workload == "combined":
+        specs = slm.workloads.combined()
+        res = batch_leg(slm, specs, "bf16", args.steps, args.warmup, args.resort,
+                        args.graph_chunk, args.sort_cell_log2, "combined")
+        out = {
+            "metric": METRIC, "value": res["agent_updates_per_s"], "unit": "agent-updates/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16", "data": "synthetic (seeded)",
+            "config": {"workload": "combined: 1024 mixed instances + 16384x16384 / 2^28 agents, "
+                                   "bf16 trail, one agent + one trail launch per step"},
+            "combined": res, "cpu_baseline": None, "roofline": None,
+        }
+        print(json.dumps(out))
+        return 0
+
+    plain = plain_leg(slm, args)
+    W, H, N = args.width, args.height, 1 << args.agents_log2
+    dominant = plain["agents_roof"] if plain["agents_s"] >= plain["trail_s"] else plain["trail_roof"]
     out = {
         "metric": METRIC,
-        "value": N * K / elapsed,
+        "value": plain["value"],
         "unit": "agent-updates/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": plain["steps"],
         "warmup": args.warmup,
-        "ms_per_step": elapsed / K * 1e3,
+        "ms_per_step": plain["elapsed"] / plain["steps"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.storage,
         "data": "synthetic (seeded SplitMix64 agent init, zero trail)",
         "config": {
-            "workload": (f"plain {args.storage} trail {W}x{H}, 2^{args.agents_log2} agents, presets "
-                         f"{' -> '.join(PRESETS)} back to back ({'/'.join(map(str, seg))} steps), "
-                         f"seed {args.seed}, hipGraph {args.graph_chunk}-step chunks"),
+            "workload": (f"plain {args.storage} trail {W}x{H}, 2^{args.agents_log2} agents, "
+                         f"presets {' -> '.join(slm.workloads.PLAIN_PRESETS)} back to back "
+                         f"({'/'.join(map(str, plain['segments']))} steps), seed {args.seed}, "
+                         f"hipGraph {args.graph_chunk}-step chunks"),
             "width": W, "height": H, "agents": N, "storage": args.storage, "seed": args.seed,
-            "presets": PRESETS, "graph_chunk": args.graph_chunk, "resort_interval": args.resort,
-            "sort_cell_log2": args.sort_cell_log2,
+            "presets": slm.workloads.PLAIN_PRESETS, "graph_chunk": args.graph_chunk,
+            "resort_interval": args.resort, "sort_cell_log2": args.sort_cell_log2,
         },
-        "roofline": roof,
+        "roofline": dominant,
         "diffuse": {
-            "kernel": "trail_strip_kernel (deposit + decay + 3x3 diffuse)",
-            "achieved_gbps": round(t_gbps, 1),
-            "pct_hbm_peak": round(100 * t_gbps / HBM_PEAK_GBPS, 2),
-            "cells_per_s": W * H / trail_avg_s,
-            "avg_launch_us": round(trail_avg_s * 1e6, 2),
-            "bytes_per_launch": t_bytes,
-            "note": "trail of this size fits the 256 MB Infinity Cache" if t_bytes <= 256e6 else "",
+            "kernel": "trail_kernel (deposit + decay + 3x3 diffuse)",
+            "achieved_gbps": plain["trail_roof"]["achieved"],
+            "pct_hbm_peak": round(100 * plain["trail_roof"]["frac"], 2),
+            "cells_per_s": W * H / plain["trail_s"],
+            "avg_launch_us": plain["trail_roof"]["avg_launch_us"],
+            "bytes_per_launch": plain["trail_roof"]["bytes_per_launch"],
+            "note": ("trail of this size fits the 256 MB Infinity Cache"
+                     if plain["trail_roof"]["bytes_per_launch"] <= 256e6 else ""),
         },
         "agents_kernel": {
-            "avg_launch_us": round(agents_avg_s * 1e6, 2),
-            "achieved_gbps": round(a_gbps, 1),
-            "agent_updates_per_s": N / agents_avg_s,
-            "bytes_per_launch": a_bytes,
+            "avg_launch_us": plain["agents_roof"]["avg_launch_us"],
+            "achieved_gbps": plain["agents_roof"]["achieved"],
+            "agent_updates_per_s": N / plain["agents_s"],
+            "bytes_per_launch": plain["agents_roof"]["bytes_per_launch"],
         },
-        "sort_avg_launch_us": round(sort_avg_s * 1e6, 2),
-        "final_total_mass": mass,
+        "sort_avg_launch_us": round(plain["sort_s"] * 1e6, 2),
+        "final_total_mass": plain["mass"],
         "cpu_baseline": None,
     }
+    if args.batch_steps > 0:
+        specs = slm.workloads.batch_mix(args.batch_count)
+        bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.resort,
+                         args.graph_chunk, args.sort_cell_log2, "batched")
+        out["batched"] = bres
+        out["batched_vs_plain"] = round(bres["agent_updates_per_s"] / plain["value"], 3)
     if args.cpu_seconds > 0:
         import oracle
 
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        n, dt = cpu_baseline(oracle, agents0, trail0, presets[0], args.cpu_seconds, threads)
+        a0, t0 = plain["state0"]
+        n, dt = cpu_baseline(oracle, a0, t0, plain["presets"][0], args.cpu_seconds, threads)
         out["cpu_baseline"] = {
             "value": N * n / dt,
             "unit": "agent-updates/s",
             "cores": threads,
             "kind": "port",
-            "sample": (f"C oracle (OpenMP, {threads} threads) on the same {W}x{H}, {N}-agent "
-                       f"initial state, Default preset, {n} whole steps in {dt:.1f} s"),
+            "sample": (f"C oracle (OpenMP, {threads} threads) on the plain leg's {W}x{H}, "
+                       f"{N}-agent initial state, Default preset, {n} whole steps in {dt:.1f} s"),
         }
     print(json.dumps(out))
     return 0
