@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--resort", type=int, default=20, help="Morton re-sort interval (0: off)")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--sort-cell-log2", type=int, default=4, help="re-sort bin edge 2^k cells")
+    ap.add_argument("--tiled", action="store_true", help="LDS-staged tiled sensing")
     ap.add_argument("--timing-steps", type=int, default=30,
                     help="eager steps per preset for the per-kernel event timing")
     ap.add_argument("--batch-steps", type=int, default=1000, help="batched leg steps (0: skip)")
@@ -143,7 +144,7 @@ def plain_leg(slm, args):
     seg = [K // 3, K // 3, K - 2 * (K // 3)]
     sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
                          resort_interval=args.resort, graph_chunk=args.graph_chunk,
-                         sort_cell_log2=args.sort_cell_log2)
+                         sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled)
     log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
         f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}")
     state0 = (sim.read_agents(), sim.read_trail()) if args.cpu_seconds > 0 else None
@@ -169,11 +170,12 @@ def plain_leg(slm, args):
     }
 
 
-def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label):
+def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label,
+              tiled=False):
     sb = STORAGE_BYTES[storage]
     log(f"{label}: {len(specs)} instances storage={storage} creating")
     b = slm.Batch(specs, storage=storage, resort_interval=resort, graph_chunk=graph_chunk,
-                  sort_cell_log2=cell_log2)
+                  sort_cell_log2=cell_log2, tiled_sensing=tiled)
     info = b.info()
     log(f"{label}: {info.total_agents} agents, {info.total_cells} cells, "
         f"{info.device_bytes / 2**30:.1f} GiB; warmup {warmup}")
@@ -201,7 +203,7 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
         "diffuse_pct_hbm_peak": round(100 * trail_bytes(sb) * n_cells / t_s / 1e9 / HBM_PEAK_GBPS, 2),
         "agents_gbps": round(agent_bytes(sb) * n_agents / a_s / 1e9, 1),
         "resort_interval": resort, "graph_chunk": graph_chunk, "sort_cell_log2": cell_log2,
-        "total_mass_sum": float(masses.sum()),
+        "tiled_sensing": tiled, "total_mass_sum": float(masses.sum()),
     }
 
 
@@ -224,7 +226,7 @@ def main():
     if args.workload == "combined":
         specs = slm.workloads.combined()
         res = batch_leg(slm, specs, "bf16", args.steps, args.warmup, args.resort,
-                        args.graph_chunk, args.sort_cell_log2, "combined")
+                        args.graph_chunk, args.sort_cell_log2, "combined", args.tiled)
         out = {
             "metric": METRIC, "value": res["agent_updates_per_s"], "unit": "agent-updates/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -261,6 +263,7 @@ def main():
             "width": W, "height": H, "agents": N, "storage": args.storage, "seed": args.seed,
             "presets": slm.workloads.PLAIN_PRESETS, "graph_chunk": args.graph_chunk,
             "resort_interval": args.resort, "sort_cell_log2": args.sort_cell_log2,
+            "tiled_sensing": args.tiled,
         },
         "roofline": dominant,
         "diffuse": {
@@ -286,7 +289,7 @@ def main():
     if args.batch_steps > 0:
         specs = slm.workloads.batch_mix(args.batch_count)
         bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.resort,
-                         args.graph_chunk, args.sort_cell_log2, "batched")
+                         args.graph_chunk, args.sort_cell_log2, "batched", args.tiled)
         out["batched"] = bres
         out["batched_vs_plain"] = round(bres["agent_updates_per_s"] / plain["value"], 3)
     if args.cpu_seconds > 0:

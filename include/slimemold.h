@@ -100,6 +100,11 @@ typedef struct slm_config {
  * the shader's f32 products; they are added to the f64 position, and the wrap,
  * floor and deposit cell are taken in f64.  Agents are still exchanged as f32. */
 #define SLM_FLAG_F64_POSITIONS 0x2u
+/* With re-sorting on, stage each 64x64-cell tile's trail window (tile plus
+ * sensor reach plus drift since the re-sort) in LDS and sense from there; taps
+ * outside the window read global memory, so results are unchanged.  Pays off
+ * for short sensor distances and frequent re-sorts. */
+#define SLM_FLAG_TILED_SENSING 0x4u
 
 typedef struct slm_info {
   uint32_t width, height;
@@ -184,8 +189,8 @@ int slm_reassign_speeds(slm_sim* sim, float speed_min, float speed_max);
 int slm_read_agents(slm_sim* sim, float* agents_xyzw, uint64_t n);
 int slm_write_agents(slm_sim* sim, const float* agents_xyzw, uint64_t n);
 
-/* Copy the trail out/in as width*height f32.  Written values must be finite
- * and >= 0 (-0 is stored as +0); 16-bit storage rounds them to nearest-even. */
+/* Copy the trail out/in as width*height f32.  Written values must lie in
+ * [0, 2^100] (-0 is stored as +0); 16-bit storage rounds them to nearest-even. */
 int slm_read_trail(slm_sim* sim, float* trail, uint64_t cells);
 int slm_write_trail(slm_sim* sim, const float* trail, uint64_t cells);
 

@@ -43,6 +43,8 @@ struct TrailRead;
 template <>
 struct TrailRead<SLM_STORAGE_F32> {
   using elem = uint32_t;
+  static constexpr uint32_t kSign = 0x80000000u;
+  __device__ static float value(uint32_t raw) { return __uint_as_float(raw); }
   __device__ static void load_pair(const void* t, uint32_t i, float& a, float& b) {
     uint2 v;
     __builtin_memcpy(&v, static_cast<const uint32_t*>(t) + i, sizeof v);
@@ -71,6 +73,8 @@ __device__ inline void mark16(void* t, uint32_t i) {
 template <>
 struct TrailRead<SLM_STORAGE_BF16> {
   using elem = uint16_t;
+  static constexpr uint32_t kSign = 0x8000u;
+  __device__ static float value(uint32_t raw) { return __uint_as_float(raw << 16); }
   __device__ static void load_pair(const void* t, uint32_t i, float& a, float& b) {
     uint32_t w = load_u16_pair(t, i);
     a = __uint_as_float((w & 0x7fffu) << 16);
@@ -82,6 +86,10 @@ struct TrailRead<SLM_STORAGE_BF16> {
 template <>
 struct TrailRead<SLM_STORAGE_F16> {
   using elem = uint16_t;
+  static constexpr uint32_t kSign = 0x8000u;
+  __device__ static float value(uint32_t raw) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(raw)));
+  }
   __device__ static void load_pair(const void* t, uint32_t i, float& a, float& b) {
     uint32_t w = load_u16_pair(t, i);
     a = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0x7fffu)));
@@ -98,13 +106,19 @@ __device__ inline float wgsl_mix(float a, float b, float t) {
   float r = b * t;
   return l + r;
 }
+// WGSL '%' for b > 0.  When |a| < b, a/b truncates to +-0 and a - b*(+-0) is
+// a itself (+0 for a = -0, which a + 0 gives), so the division is skipped;
+// checked against the full formula for every float within 4b of 0 for 16
+// moduli.
 __device__ inline float wgsl_mod(float a, float b) {
+  if (__builtin_fabsf(a) < b) return a + 0.0f;
   float q = a / b;
   float t = __builtin_truncf(q);
   float p = b * t;
   return a - p;
 }
 __device__ inline double mod64(double a, double b) {
+  if (__builtin_fabs(a) < b) return a + 0.0;
   double q = a / b;
   double t = __builtin_trunc(q);
   double p = b * t;
@@ -140,21 +154,76 @@ __device__ inline float sample_at(const void* t, uint32_t w, uint32_t h, int x0,
   return in ? v : 0.0f;
 }
 
+// Taps straight from the trail in global memory.
 template <int S>
-__device__ inline float sample(const void* t, uint32_t w, uint32_t h, float px, float py) {
+struct GlobalTaps {
+  void* t;
+  uint32_t w, h;
+  __device__ float sample(int x0, int y0, float dx, float dy) const {
+    return sample_at<S>(t, w, h, x0, y0, dx, dy);
+  }
+  __device__ void mark(uint32_t x, uint32_t y) const { TrailRead<S>::mark(t, y * w + x); }
+};
+
+// Taps from a window of the trail staged in LDS (|T| values of cells
+// [wx0, wx0+ww) x [wy0, wy0+wh)); taps outside it go to global memory, so
+// the result does not depend on the window.
+template <int S>
+struct WindowTaps {
+  using elem = typename TrailRead<S>::elem;
+  void* t;
+  uint32_t w, h;
+  const elem* win;
+  int wx0, wy0, ww, wh;
+  __device__ float sample(int x0, int y0, float dx, float dy) const {
+    bool in = x0 >= 0 && static_cast<uint32_t>(x0) < w - 1u && y0 >= 0 &&
+              static_cast<uint32_t>(y0) < h - 1u;
+    int lx = x0 - wx0, ly = y0 - wy0;
+    bool inw = in && lx >= 0 && lx + 1 < ww && ly >= 0 && ly + 1 < wh;
+    float v00, v10, v01, v11;
+    if (inw) {
+      int i = ly * ww + lx;
+      v00 = TrailRead<S>::value(win[i]);
+      v10 = TrailRead<S>::value(win[i + 1]);
+      v01 = TrailRead<S>::value(win[i + ww]);
+      v11 = TrailRead<S>::value(win[i + ww + 1]);
+    } else {
+      uint32_t i00 = in ? static_cast<uint32_t>(y0) * w + static_cast<uint32_t>(x0) : 0u;
+      uint32_t down = in ? w : 0u;
+      TrailRead<S>::load_pair(t, i00, v00, v10);
+      TrailRead<S>::load_pair(t, i00 + down, v01, v11);
+    }
+    float v0 = wgsl_mix(v00, v10, dx);
+    float v1 = wgsl_mix(v01, v11, dx);
+    float v = wgsl_mix(v0, v1, dy);
+    return in ? v : 0.0f;
+  }
+  // Inside the window |T| is known, so the mark is a store only.
+  __device__ void mark(uint32_t x, uint32_t y) const {
+    int lx = static_cast<int>(x) - wx0, ly = static_cast<int>(y) - wy0;
+    if (lx >= 0 && lx < ww && ly >= 0 && ly < wh)
+      static_cast<elem*>(t)[y * w + x] =
+          static_cast<elem>(static_cast<uint32_t>(win[ly * ww + lx]) | TrailRead<S>::kSign);
+    else
+      TrailRead<S>::mark(t, y * w + x);
+  }
+};
+
+template <class Taps>
+__device__ inline float sample(const Taps& taps, float px, float py) {
   int x0 = static_cast<int>(__builtin_floorf(px));
   int y0 = static_cast<int>(__builtin_floorf(py));
   float dx = px - static_cast<float>(x0);
   float dy = py - static_cast<float>(y0);
-  return sample_at<S>(t, w, h, x0, y0, dx, dy);
+  return taps.sample(x0, y0, dx, dy);
 }
 
-template <int S>
-__device__ inline float sample64(const void* t, uint32_t w, uint32_t h, double px, double py) {
+template <class Taps>
+__device__ inline float sample64(const Taps& taps, double px, double py) {
   double fx = __builtin_floor(px), fy = __builtin_floor(py);
   float dx = static_cast<float>(px - fx);
   float dy = static_cast<float>(py - fy);
-  return sample_at<S>(t, w, h, static_cast<int>(fx), static_cast<int>(fy), dx, dy);
+  return taps.sample(static_cast<int>(fx), static_cast<int>(fy), dx, dy);
 }
 
 // compute.wgsl:98-112: turn towards the strongest of left/centre/right.
@@ -182,23 +251,39 @@ __device__ inline float jitter_term(uint32_t index, float x, float y, float jitt
   return (rnd * 2.0f - 1.0f) * jitter;
 }
 
-// One agent (slot j of instance `in`) for one step.
-template <int S, bool P64>
-__device__ inline void agent_step(const AgentSoA& a, uint32_t j, const Instance& in,
-                                  const DevParams& p, void* trail) {
-  const uint32_t w = in.width, h = in.height;
-  float angle = a.th[j], speed = a.v[j];
-  float x, y;
-  double xd = 0.0, yd = 0.0;
+// An agent's state as loaded at the start of its update.
+template <bool P64>
+struct AgentState {
+  float angle, speed, x, y;
+  double xd, yd;
+};
+
+template <bool P64>
+__device__ inline AgentState<P64> load_agent(const AgentSoA& a, uint32_t j) {
+  AgentState<P64> st;
+  st.angle = a.th[j];
+  st.speed = a.v[j];
   if (P64) {
-    xd = a.x64[j];
-    yd = a.y64[j];
-    x = static_cast<float>(xd);
-    y = static_cast<float>(yd);
+    st.xd = a.x64[j];
+    st.yd = a.y64[j];
+    st.x = static_cast<float>(st.xd);
+    st.y = static_cast<float>(st.yd);
   } else {
-    x = a.x[j];
-    y = a.y[j];
+    st.xd = st.yd = 0.0;
+    st.x = a.x[j];
+    st.y = a.y[j];
   }
+  return st;
+}
+
+// One agent (slot j of instance `in`, state st) for one step.
+template <bool P64, class Taps>
+__device__ inline void agent_update(const AgentSoA& a, uint32_t j, const AgentState<P64>& st,
+                                    const Instance& in, const DevParams& p, const Taps& taps) {
+  const uint32_t w = in.width, h = in.height;
+  float angle = st.angle, speed = st.speed;
+  float x = st.x, y = st.y;
+  double xd = st.xd, yd = st.yd;
 
   float sp = wgsl_clamp(speed, p.speed_min, p.speed_max);  // :72
   if (sp != speed) a.v[j] = sp;
@@ -212,16 +297,13 @@ __device__ inline void agent_step(const AgentSoA& a, uint32_t j, const Instance&
   float sd = p.sensor_dist;
   float lv, rv, cv;
   if (P64) {
-    lv = sample64<S>(trail, w, h, xd + static_cast<double>(sd * cl),
-                     yd + static_cast<double>(sd * sl));
-    rv = sample64<S>(trail, w, h, xd + static_cast<double>(sd * cr),
-                     yd + static_cast<double>(sd * sr));
-    cv = sample64<S>(trail, w, h, xd + static_cast<double>(sd * cc),
-                     yd + static_cast<double>(sd * sc));
+    lv = sample64(taps, xd + static_cast<double>(sd * cl), yd + static_cast<double>(sd * sl));
+    rv = sample64(taps, xd + static_cast<double>(sd * cr), yd + static_cast<double>(sd * sr));
+    cv = sample64(taps, xd + static_cast<double>(sd * cc), yd + static_cast<double>(sd * sc));
   } else {
-    lv = sample<S>(trail, w, h, x + sd * cl, y + sd * sl);  // :79-95, read order L, R, C
-    rv = sample<S>(trail, w, h, x + sd * cr, y + sd * sr);
-    cv = sample<S>(trail, w, h, x + sd * cc, y + sd * sc);
+    lv = sample(taps, x + sd * cl, y + sd * sl);  // :79-95, read order L, R, C
+    rv = sample(taps, x + sd * cr, y + sd * sr);
+    cv = sample(taps, x + sd * cc, y + sd * sc);
   }
   angle = turn(angle, lv, rv, cv, p.turn);
 
@@ -266,7 +348,33 @@ __device__ inline void agent_step(const AgentSoA& a, uint32_t j, const Instance&
   }
   a.th[j] = angle;
   if (dx >= 0 && static_cast<uint32_t>(dx) < w && dy >= 0 && static_cast<uint32_t>(dy) < h)
-    TrailRead<S>::mark(trail, static_cast<uint32_t>(dy) * w + static_cast<uint32_t>(dx));  // :136-141
+    taps.mark(static_cast<uint32_t>(dx), static_cast<uint32_t>(dy));  // :136-141
+}
+
+template <bool P64, class Taps>
+__device__ inline void agent_step(const AgentSoA& a, uint32_t j, const Instance& in,
+                                  const DevParams& p, const Taps& taps) {
+  agent_update<P64>(a, j, load_agent<P64>(a, j), in, p, taps);
+}
+
+// Agents begin + threadIdx.x, + block, ... below end, with the next agent's
+// state loaded while the current one is computed.
+template <bool P64, int kBlock, class Taps>
+__device__ inline void agent_loop(const AgentSoA& a, uint32_t begin, uint32_t end,
+                                  const Instance& in, const DevParams& p, const Taps& taps) {
+  uint32_t j = begin + threadIdx.x;
+  if (j >= end) return;
+  AgentState<P64> cur = load_agent<P64>(a, j);
+  for (;;) {
+    uint32_t nj = j + kBlock;
+    bool more = nj < end;
+    AgentState<P64> nxt = cur;
+    if (more) nxt = load_agent<P64>(a, nj);
+    agent_update<P64>(a, j, cur, in, p, taps);
+    if (!more) break;
+    j = nj;
+    cur = nxt;
+  }
 }
 
 // Workgroups of one launch are dealt round-robin to the 8 XCDs; give XCD k a
@@ -300,10 +408,112 @@ __global__ __launch_bounds__(kAgentBlock) void agents_kernel(AgentArgs args) {
     uint32_t end = in.agent_off + in.agent_count;
     if (j >= in.agent_off && j < end) {
       const DevParams p = args.params[ii];
-      agent_step<S, P64>(args.a, j, in, p, static_cast<elem*>(args.trail) + in.trail_off);
+      GlobalTaps<S> taps{static_cast<elem*>(args.trail) + in.trail_off, in.width, in.height};
+      agent_step<P64>(args.a, j, in, p, taps);
     }
     if (last < end || ii + 1u >= args.count) break;
   }
+}
+
+// ---- tiled sensing -----------------------------------------------------------
+// After a re-sort the agents of each 2^k x 2^k-cell tile occupy consecutive
+// slots (the tile is an aligned square of Morton bins).  One workgroup takes a
+// work item (a tile's slots, at most kItemAgents) and stages the trail around
+// the tile - the tile plus a margin covering the sensor distance and the
+// distance agents may have drifted since the re-sort - in LDS once, so sensing
+// reads LDS instead of issuing divergent global loads (which kept the texture
+// address unit ~97% busy).  Taps that still fall outside use global memory.
+constexpr int kTiledBlock = 1024;
+
+// Largest square window of `elem_bytes` cells that fits kWindowBytes.
+constexpr int window_side_of(size_t elem_bytes) {
+  int side = 1;
+  while (static_cast<size_t>(side + 1) * (side + 1) * elem_bytes <= kWindowBytes) side++;
+  return side;
+}
+constexpr uint32_t kMinStagedAgents = 256;  // sparser tiles read global memory
+
+__device__ inline uint32_t compact_bits(uint32_t v) {  // even bits -> 16 bits
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0f0f0f0fu;
+  v = (v | (v >> 4)) & 0x00ff00ffu;
+  v = (v | (v >> 8)) & 0x0000ffffu;
+  return v;
+}
+
+struct TiledArgs {
+  AgentSoA a;
+  void* trail;
+  const Instance* inst;
+  const DevParams* params;
+  const WorkItem* items;
+  const uint32_t* item_total;
+  uint32_t side;        // LDS window side limit (cells)
+  uint32_t resort;      // steps between re-sorts
+  int xcd_remap;
+};
+
+template <int S, bool P64>
+__global__ __launch_bounds__(kTiledBlock) void agents_tiled_kernel(TiledArgs args) {
+  using elem = typename TrailRead<S>::elem;
+  extern __shared__ __align__(16) unsigned char smem[];
+  elem* win = reinterpret_cast<elem*>(smem);
+  uint32_t item = args.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  if (item >= *args.item_total) return;  // uniform per block
+  const WorkItem wi = args.items[item];
+  const Instance in = args.inst[wi.inst];
+  const DevParams p = args.params[wi.inst];
+  elem* trail = static_cast<elem*>(args.trail) + in.trail_off;
+  const int tile = 1 << in.tile_log2;
+  const int tx = static_cast<int>(compact_bits(wi.tile)) * tile;
+  const int ty = static_cast<int>(compact_bits(wi.tile >> 1)) * tile;
+  // margin the window must cover: sensor reach + bilinear neighbour + drift
+  float reach = __builtin_ceilf(__builtin_fabsf(p.sensor_dist)) + 2.0f +
+                __builtin_ceilf(p.speed_max * kTimeStep * static_cast<float>(args.resort));
+  int room = (static_cast<int>(args.side) - tile) / 2;
+  bool staged = room > 0 && reach <= static_cast<float>(room) &&
+                wi.end - wi.begin >= kMinStagedAgents;  // uniform
+  if (!staged) {
+    GlobalTaps<S> taps{trail, in.width, in.height};
+    agent_loop<P64, kTiledBlock>(args.a, wi.begin, wi.end, in, p, taps);
+    return;
+  }
+  const int m = room;
+  const int wx0 = max(tx - m, 0), wx1 = min(tx + tile + m, static_cast<int>(in.width));
+  const int wy0 = max(ty - m, 0), wy1 = min(ty + tile + m, static_cast<int>(in.height));
+  const int ww = wx1 - wx0, wh = wy1 - wy0;
+  // Stage the window: wave w takes rows w, w + 16, ..., lane l columns l,
+  // l + 64, ...; every thread issues all of its loads before its LDS stores,
+  // so they are in flight together.
+  constexpr int kSide = window_side_of(sizeof(elem));
+  constexpr int kWaves = kTiledBlock / 64;
+  constexpr int kRows = (kSide + kWaves - 1) / kWaves;
+  constexpr int kCols = (kSide + 63) / 64;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t v[kRows][kCols];
+#pragma unroll
+  for (int rr = 0; rr < kRows; rr++) {
+    int r = wave + rr * kWaves;
+    const elem* row = trail + static_cast<size_t>(wy0 + r) * in.width + wx0;
+#pragma unroll
+    for (int cc = 0; cc < kCols; cc++) {
+      int c = lane + cc * 64;
+      v[rr][cc] = (r < wh && c < ww) ? static_cast<uint32_t>(row[c]) : 0u;
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < kRows; rr++) {
+    int r = wave + rr * kWaves;
+#pragma unroll
+    for (int cc = 0; cc < kCols; cc++) {
+      int c = lane + cc * 64;
+      if (r < wh && c < ww) win[r * ww + c] = static_cast<elem>(v[rr][cc] & ~TrailRead<S>::kSign);
+    }
+  }
+  __syncthreads();
+  WindowTaps<S> taps{trail, in.width, in.height, win, wx0, wy0, ww, wh};
+  agent_loop<P64, kTiledBlock>(args.a, wi.begin, wi.end, in, p, taps);
 }
 
 // ---- initialisation ------------------------------------------------------
@@ -421,6 +631,33 @@ void launch_agents(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t
       launch_agents_t<SLM_STORAGE_F32>(pool, args, stream);
       break;
   }
+}
+
+uint32_t window_side(int storage) { return window_side_of(storage_bytes(storage)); }
+
+void launch_agents_tiled(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t stream) {
+  if (pool.total_agents == 0 || pool.max_items == 0) return;
+  TiledArgs args{a,          trail,           pool.inst,    pool.params, pool.items,
+                 pool.item_total, window_side(pool.storage), pool.resort_interval, pool.xcd_remap};
+  size_t lds = kWindowBytes;
+  uint32_t nb = pool.max_items;
+#define SLM_TILED(S)                                                                 \
+  if (pool.pos64)                                                                    \
+    agents_tiled_kernel<S, true><<<nb, kTiledBlock, lds, stream>>>(args);            \
+  else                                                                               \
+    agents_tiled_kernel<S, false><<<nb, kTiledBlock, lds, stream>>>(args)
+  switch (pool.storage) {
+    case SLM_STORAGE_BF16:
+      SLM_TILED(SLM_STORAGE_BF16);
+      break;
+    case SLM_STORAGE_F16:
+      SLM_TILED(SLM_STORAGE_F16);
+      break;
+    default:
+      SLM_TILED(SLM_STORAGE_F32);
+      break;
+  }
+#undef SLM_TILED
 }
 
 void launch_init_agents(const Instance& in, const AgentSoA& a, int pos64, uint64_t seed,

@@ -60,8 +60,10 @@ constexpr uint64_t kMaxPoolCells = 1ull << 40;
 constexpr uint64_t kMaxAgents = 0xffffff00ull;    // u32 slot index, whole blocks
 constexpr float kMaxSensorDistance = 4194304.0f;  // keeps sensor taps inside i32
 constexpr uint64_t kMaxTotalBins = 1ull << 30;
+constexpr float kMaxTrailValue = 0x1p100f;        // keeps every 3x3 window sum finite
 constexpr uint32_t kTargetWaves = 8192;           // trail-pass units per pool
-constexpr uint32_t kKnownFlags = SLM_FLAG_NO_XCD_REMAP | SLM_FLAG_F64_POSITIONS;
+constexpr uint32_t kKnownFlags =
+    SLM_FLAG_NO_XCD_REMAP | SLM_FLAG_F64_POSITIONS | SLM_FLAG_TILED_SENSING;
 
 bool finite_params(const slm_params& p) {
   const float* f = &p.decay_factor;
@@ -170,6 +172,12 @@ class Engine {
     p.pos64 = pos64() ? 1 : 0;
     p.xcd_remap = (cfg.flags & SLM_FLAG_NO_XCD_REMAP) ? 0 : 1;
     p.sort_cell_log2 = cfg.sort_cell_log2;
+    p.total_tiles = total_tiles_;
+    p.tile_inst = tile_inst_d_;
+    p.items = items_d_;
+    p.item_total = item_total_d_;
+    p.max_items = max_items_;
+    p.resort_interval = cfg.resort_interval;
     return p;
   }
 
@@ -204,8 +212,9 @@ class Engine {
         (void)hipEventRecord(ev[1], stream);
         acur_ ^= 1;
       }
+      if (sort) tiles_ready_ = true;
       (void)hipEventRecord(ev[2], stream);
-      launch_agents(p, agents_[acur_], trail_[cur_], stream);
+      enqueue_agents(p);
       (void)hipEventRecord(ev[3], stream);
       launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
       (void)hipEventRecord(ev[4], stream);
@@ -323,8 +332,8 @@ class Engine {
     uint64_t c = cells(k);
     std::vector<float> clean(in, in + c);
     for (uint64_t i = 0; i < c; i++) {
-      if (!(clean[i] >= 0.0f) || !std::isfinite(clean[i]))
-        return fail(SLM_ERR_INVALID_ARGUMENT, "trail cell %llu must be finite and >= 0",
+      if (!(clean[i] >= 0.0f) || !(clean[i] <= kMaxTrailValue))
+        return fail(SLM_ERR_INVALID_ARGUMENT, "trail cell %llu must lie in [0, 2^100]",
                     static_cast<unsigned long long>(i));
       if (clean[i] == 0.0f) clean[i] = 0.0f;  // -0 -> +0: the sign bit is the deposit mark
     }
@@ -392,6 +401,11 @@ class Engine {
   double* mass_partials_ = nullptr;
   double* mass_out_ = nullptr;
   uint32_t total_agents_ = 0, total_units_ = 0, total_bins_ = 0;
+  uint32_t total_tiles_ = 0, max_items_ = 0;
+  uint32_t* tile_inst_d_ = nullptr;
+  WorkItem* items_d_ = nullptr;
+  uint32_t* item_total_d_ = nullptr;
+  bool tiles_ready_ = false;  // the latest re-sort's work list matches the layout
   uint64_t trail_elems_ = 0;
   hipGraphExec_t graphs_[2][2] = {};
   std::vector<std::pair<void*, size_t>> allocs_;
@@ -576,7 +590,14 @@ class Engine {
       plan_bins(in, cl);
       in.bin_off = static_cast<uint32_t>(bins);
       bins += in.nbins;
+      plan_tiles(in, cl, kTileLog2);
     }
+    uint64_t tiles = 0;
+    for (auto& in : inst) {
+      in.tile_off = static_cast<uint32_t>(tiles);
+      tiles += in.tiles;
+    }
+    tiles_ready_ = false;
     if (units >= (1ull << 32)) return fail(SLM_ERR_INVALID_ARGUMENT, "too many trail units");
     if (sorting() && bins > kMaxTotalBins)
       return fail(SLM_ERR_INVALID_ARGUMENT, "too many re-sort bins: raise sort_cell_log2");
@@ -599,11 +620,31 @@ class Engine {
 
     dev_free(sort_.count);
     dev_free(sort_.offset);
+    dev_free(sort_.tile_items);
     dev_free(scan_levels_);
-    sort_.count = sort_.offset = scan_levels_ = nullptr;
+    dev_free(tile_inst_d_);
+    dev_free(items_d_);
+    dev_free(item_total_d_);
+    sort_.count = sort_.offset = sort_.tile_items = scan_levels_ = nullptr;
+    tile_inst_d_ = item_total_d_ = nullptr;
+    items_d_ = nullptr;
+    total_tiles_ = max_items_ = 0;
     if (sorting()) {
       SLM_TRY(dev_alloc(&sort_.count, total_bins_ * 4ull));
-      SLM_TRY(dev_alloc(&sort_.offset, total_bins_ * 4ull));
+      SLM_TRY(dev_alloc(&sort_.offset, (total_bins_ + 1ull) * 4ull));
+      // offset[total_bins] closes the last bin: the scan fills the rest
+      SLM_HIP(hipMemcpy(sort_.offset + total_bins_, &total_agents_, 4, hipMemcpyHostToDevice));
+      total_tiles_ = static_cast<uint32_t>(tiles);
+      max_items_ = total_tiles_ + blocks_for(total_agents_, kItemAgents);
+      std::vector<uint32_t> tmap(total_tiles_);
+      for (size_t k = 0; k < spec.size(); k++)
+        for (uint32_t t = 0; t < inst[k].tiles; t++) tmap[inst[k].tile_off + t] = static_cast<uint32_t>(k);
+      SLM_TRY(dev_alloc(&tile_inst_d_, tmap.size() * 4ull));
+      SLM_HIP(hipMemcpy(tile_inst_d_, tmap.data(), tmap.size() * 4ull, hipMemcpyHostToDevice));
+      SLM_TRY(dev_alloc(&sort_.tile_items, total_tiles_ * 4ull));
+      SLM_TRY(dev_alloc(&items_d_, max_items_ * sizeof(WorkItem)));
+      SLM_TRY(dev_alloc(&item_total_d_, 4));
+      SLM_HIP(hipMemset(item_total_d_, 0, 4));
       uint32_t lv = plan_scan(sort_, total_bins_);
       SLM_TRY(dev_alloc(&scan_levels_, lv * 4ull));
       uint32_t at = 0;
@@ -630,13 +671,23 @@ class Engine {
     return sorting() && step_index % cfg.resort_interval == 0;
   }
 
+  // Tiled sensing (when asked for) once a re-sort has laid agents out by
+  // tile; otherwise the global-memory pass.  Both give the same bits.
+  void enqueue_agents(const Pool& p) {
+    if (sorting() && tiles_ready_ && (cfg.flags & SLM_FLAG_TILED_SENSING))
+      launch_agents_tiled(p, agents_[acur_], trail_[cur_], stream);
+    else
+      launch_agents(p, agents_[acur_], trail_[cur_], stream);
+  }
+
   void enqueue_step(bool do_sort) {
     Pool p = pool();
     if (do_sort && sorting() && total_agents_ > 0) {
       launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
       acur_ ^= 1;
+      tiles_ready_ = true;
     }
-    launch_agents(p, agents_[acur_], trail_[cur_], stream);
+    enqueue_agents(p);
     launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
     cur_ ^= 1;
   }

@@ -57,7 +57,16 @@ struct Instance {
   uint32_t bin_off;                 // re-sort bins [bin_off, bin_off + nbins)
   uint32_t nbins;
   uint32_t nbx, nby;                // bins across / down
+  uint32_t tile_log2;               // sensing tiles of 2^k x 2^k cells (aligned bin squares)
+  uint32_t tile_off, tiles;         // tiles [tile_off, tile_off + tiles), Morton order
 };
+
+// One workgroup's share of the tiled agent pass: agent slots [begin, end) of
+// tile `tile` (instance-local Morton index) of instance `inst`.
+struct WorkItem {
+  uint32_t inst, tile, begin, end;
+};
+constexpr uint32_t kItemAgents = 4096;  // agents per work item at most
 
 // A launchable description of a pool, with device copies of its tables.
 struct Pool {
@@ -73,7 +82,16 @@ struct Pool {
   int pos64;      // f64 positions
   int xcd_remap;  // group consecutive logical blocks on one XCD
   uint32_t sort_cell_log2;
+  // tiled sensing (valid after a re-sort has built the work list)
+  uint32_t total_tiles;
+  const uint32_t* tile_inst;   // device: instance of each tile
+  WorkItem* items;             // device: work list, at most max_items entries
+  uint32_t* item_total;        // device: number of valid work items
+  uint32_t max_items;
+  uint32_t resort_interval;    // steps agents may drift from their tile between re-sorts
 };
+
+constexpr uint32_t kTileLog2 = 6;  // sensing tiles of 64 x 64 cells
 
 constexpr int kAgentBlock = 256;
 constexpr int kTrailBlock = 256;
@@ -86,9 +104,18 @@ inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t
 void plan_trail(Instance& in, int storage, uint32_t target_waves);
 // Host-side layout of one instance's re-sort bins: fills nbins, nbx, nby.
 void plan_bins(Instance& in, uint32_t cell_log2);
+// Tiles of 2^tile_log2 cells (>= the bin edge) for tiled sensing: fills
+// tile_log2 and tiles.
+void plan_tiles(Instance& in, uint32_t cell_log2, uint32_t tile_log2);
 
 // ---- launchers (all asynchronous on `stream`) ----------------------------
 void launch_agents(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t stream);
+// The same update with each tile's trail window staged in LDS (needs the work
+// list of the latest re-sort); bit-identical to launch_agents.
+void launch_agents_tiled(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t stream);
+// LDS bytes the tiled pass stages per workgroup for this storage format.
+constexpr uint32_t kWindowBytes = 65536;
+uint32_t window_side(int storage);
 void launch_trail(const Pool& pool, const void* trail_in, void* trail_out, hipStream_t stream);
 void launch_set_params(DevParams* dst, const DevParams& src, hipStream_t stream);
 void launch_init_agents(const Instance& in, const AgentSoA& a, int pos64, uint64_t seed,
@@ -116,7 +143,8 @@ struct SortScratch {
   uint32_t* key;       // per slot: global bin
   uint32_t* rank;      // per slot: rank inside its bin
   uint32_t* count;     // per bin
-  uint32_t* offset;    // per bin, exclusive prefix sum
+  uint32_t* offset;    // per bin, exclusive prefix sum; offset[total_bins] = total agents
+  uint32_t* tile_items;  // per tile: work items, then their exclusive prefix sum
   uint32_t* level[3];  // tile sums of the multi-level scan
   uint32_t level_len[3];
   int levels;

@@ -41,14 +41,24 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
-// Bins each agent and takes its rank inside the bin.  Lanes of a wave that share
-// a bin are served by one atomic (agents re-sorted earlier arrive grouped by
-// bin, so a wave usually holds one to four bins); ranks follow lane order.
-__global__ void count_kernel(AgentSoA a, const Instance* insts, const uint32_t* block_inst,
-                             uint32_t ninst, uint32_t total, int pos64, uint32_t cell_log2,
-                             uint32_t* key, uint32_t* rank, uint32_t* count) {
-  uint32_t first = blockIdx.x * blockDim.x;
-  uint32_t last = min(first + blockDim.x, total) - 1u;
+// Bins each agent and takes its rank inside the bin, with as few global
+// atomics as possible: agents re-sorted earlier arrive grouped by bin, so a
+// block's 256 agents usually span a handful of consecutive bins.  Then the
+// lanes of each wave that share a bin are counted with one LDS atomic, and
+// one global atomic per distinct bin of the block reserves the block's range.
+// (Dense instances put ~10^4 agents in one bin; a global atomic per wave and
+// bin serialised ~10^2 times more.)  Blocks whose bins span a wider range fall
+// back to one global atomic per wave and bin.  Ranks are not in a fixed order.
+constexpr uint32_t kLocalBins = 1024;
+
+__global__ __launch_bounds__(kAgentBlock) void count_kernel(
+    AgentSoA a, const Instance* insts, const uint32_t* block_inst, uint32_t ninst, uint32_t total,
+    int pos64, uint32_t cell_log2, uint32_t* key, uint32_t* rank, uint32_t* count) {
+  __shared__ uint32_t s_min, s_max;
+  __shared__ uint32_t s_cnt[kLocalBins];
+  __shared__ uint32_t s_base[kLocalBins];
+  uint32_t first = blockIdx.x * kAgentBlock;
+  uint32_t last = min(first + kAgentBlock, total) - 1u;
   uint32_t j = first + threadIdx.x;
   bool active = j < total;
   uint32_t k = 0;
@@ -62,6 +72,22 @@ __global__ void count_kernel(AgentSoA a, const Instance* insts, const uint32_t* 
     }
     if (last < end || ii + 1u >= ninst) break;
   }
+  if (threadIdx.x == 0) {
+    s_min = 0xffffffffu;
+    s_max = 0u;
+  }
+  __syncthreads();
+  if (active) {
+    atomicMin(&s_min, k);
+    atomicMax(&s_max, k);
+  }
+  __syncthreads();
+  const uint32_t kmin = s_min, kmax = s_max;
+  const bool local = kmax >= kmin && kmax - kmin < kLocalBins;  // block-uniform
+  if (local)
+    for (uint32_t i = threadIdx.x; i <= kmax - kmin; i += kAgentBlock) s_cnt[i] = 0u;
+  __syncthreads();
+
   uint32_t lane = threadIdx.x & 63u;
   uint64_t below = (uint64_t(1) << lane) - 1u;
   uint64_t pending = __ballot(active);
@@ -71,10 +97,21 @@ __global__ void count_kernel(AgentSoA a, const Instance* insts, const uint32_t* 
     uint32_t lk = __shfl(k, leader);
     uint64_t same = __ballot(active && k == lk);
     uint32_t base = 0;
-    if (static_cast<int>(lane) == leader) base = atomicAdd(count + lk, __popcll(same));
+    if (static_cast<int>(lane) == leader)
+      base = local ? atomicAdd(&s_cnt[lk - kmin], __popcll(same))
+                   : atomicAdd(count + lk, __popcll(same));
     base = __shfl(base, leader);
     if (active && k == lk) r = base + __popcll(same & below);
     pending &= ~same;
+  }
+  if (local) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i <= kmax - kmin; i += kAgentBlock) {
+      uint32_t c = s_cnt[i];
+      if (c) s_base[i] = atomicAdd(count + kmin + i, c);
+    }
+    __syncthreads();
+    if (active) r += s_base[k - kmin];
   }
   if (active) {
     key[j] = k;
@@ -138,6 +175,39 @@ __global__ void scatter_kernel(AgentSoA src, AgentSoA dst, uint32_t n, int pos64
   dst.id[d] = src.id[j];
 }
 
+// Work items of each tile: ceil(agents / kItemAgents), 0 for an empty tile.
+__global__ void tile_count_kernel(const Instance* insts, const uint32_t* tile_inst,
+                                  uint32_t ntiles, uint32_t cell_log2, const uint32_t* offset,
+                                  uint32_t* tile_items) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntiles) return;
+  const Instance in = insts[tile_inst[g]];
+  uint32_t t = g - in.tile_off;
+  uint32_t per = 1u << (2u * (in.tile_log2 - cell_log2));  // bins per tile
+  uint32_t b0 = in.bin_off + t * per;
+  uint32_t n = offset[b0 + per] - offset[b0];
+  tile_items[g] = (n + kItemAgents - 1u) / kItemAgents;
+}
+
+// Writes each tile's work items at its scanned offset; the last tile also
+// writes the total.
+__global__ void tile_items_kernel(const Instance* insts, const uint32_t* tile_inst,
+                                  uint32_t ntiles, uint32_t cell_log2, const uint32_t* offset,
+                                  const uint32_t* item_off, WorkItem* items, uint32_t* total) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntiles) return;
+  uint32_t ii = tile_inst[g];
+  const Instance in = insts[ii];
+  uint32_t t = g - in.tile_off;
+  uint32_t per = 1u << (2u * (in.tile_log2 - cell_log2));
+  uint32_t b0 = in.bin_off + t * per;
+  uint32_t begin = offset[b0], end = offset[b0 + per];
+  uint32_t at = item_off[g];
+  for (uint32_t s = begin; s < end; s += kItemAgents)
+    items[at++] = WorkItem{ii, t, s, min(s + kItemAgents, end)};
+  if (g == ntiles - 1u) *total = at;
+}
+
 inline uint32_t next_pow2(uint32_t v) {
   uint32_t p = 1;
   while (p < v) p <<= 1;
@@ -178,6 +248,16 @@ uint32_t plan_scan(SortScratch& s, uint32_t nbins) {
   return total;
 }
 
+void plan_tiles(Instance& in, uint32_t cell_log2, uint32_t tile_log2) {
+  in.tile_log2 = tile_log2 > cell_log2 ? tile_log2 : cell_log2;
+  in.tiles = in.nbins >> (2u * (in.tile_log2 - cell_log2));
+  if (in.tiles == 0) {  // a whole grid smaller than one tile: one tile of all bins
+    in.tile_log2 = cell_log2;
+    while ((1u << (2u * (in.tile_log2 - cell_log2))) < in.nbins) in.tile_log2++;
+    in.tiles = 1;
+  }
+}
+
 void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& dst,
                         const SortScratch& s, hipStream_t stream) {
   if (pool.total_agents == 0) return;
@@ -188,6 +268,15 @@ void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& d
   scan(s, s.count, pool.total_bins, s.offset, 0, stream);
   scatter_kernel<<<blocks_for(pool.total_agents, 256), 256, 0, stream>>>(
       src, dst, pool.total_agents, pool.pos64, s.key, s.rank, s.offset);
+  if (pool.items) {  // work list of the tiled agent pass
+    uint32_t nt = pool.total_tiles;
+    tile_count_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(
+        pool.inst, pool.tile_inst, nt, pool.sort_cell_log2, s.offset, s.tile_items);
+    scan(s, s.tile_items, nt, s.tile_items, 0, stream);
+    tile_items_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(
+        pool.inst, pool.tile_inst, nt, pool.sort_cell_log2, s.offset, s.tile_items, pool.items,
+        pool.item_total);
+  }
 }
 
 }  // namespace slm

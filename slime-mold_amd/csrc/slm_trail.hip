@@ -44,12 +44,21 @@ struct Codec<SLM_STORAGE_F32> {
     return k == 0 ? r.x : k == 1 ? r.y : k == 2 ? r.z : r.w;
   }
   __device__ static uint4 pack(const uint32_t (&e)[4]) { return make_uint4(e[0], e[1], e[2], e[3]); }
+  __device__ static uint4 pack_values(const float (&v)[4]) {
+    return make_uint4(__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]),
+                      __float_as_uint(v[3]));
+  }
 };
 
-// Round-to-nearest-even f32 -> bfloat16 (finite inputs).
+// Round-to-nearest-even f32 -> bfloat16: gfx950's v_cvt_pk_bf16_f32.
 __device__ inline uint32_t bf16_rne(float v) {
-  uint32_t b = __float_as_uint(v);
-  return (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v));
+}
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t bf16x2_rne(float a, float b) {
+  f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
 template <>
@@ -67,6 +76,10 @@ struct Codec<SLM_STORAGE_BF16> {
     return make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16),
                       e[6] | (e[7] << 16));
   }
+  __device__ static uint4 pack_values(const float (&v)[8]) {
+    return make_uint4(bf16x2_rne(v[0], v[1]), bf16x2_rne(v[2], v[3]), bf16x2_rne(v[4], v[5]),
+                      bf16x2_rne(v[6], v[7]));
+  }
 };
 
 template <>
@@ -82,6 +95,12 @@ struct Codec<SLM_STORAGE_F16> {
   }
   __device__ static uint32_t lane(uint4 r, int k) { return Codec<SLM_STORAGE_BF16>::lane(r, k); }
   __device__ static uint4 pack(const uint32_t (&e)[8]) { return Codec<SLM_STORAGE_BF16>::pack(e); }
+  __device__ static uint4 pack_values(const float (&v)[8]) {
+    uint32_t e[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) e[k] = encode(v[k]);
+    return pack(e);
+  }
 };
 
 // Deposit (S3) and decay of one stored element.
@@ -97,9 +116,28 @@ __device__ inline float deposit_decay(uint32_t raw, const DevParams& p) {
   return u > 0.0f ? u : 0.0f;
 }
 
-// mix(centre, window/9, rate) (compute.wgsl:193-194).
+// sum / 9.0f with IEEE round-to-nearest, as a reciprocal multiply and one
+// fused correction step: equal to the division for every finite sum >= 0
+// (checked exhaustively over all 2^31 non-negative floats), about a third of
+// the division's instructions.
+__device__ inline float div9(float sum) {
+  const float inv9 = 0x1.c71c72p-4f;  // nearest float to 1/9
+  float q = sum * inv9;
+  float r = __builtin_fmaf(-q, 9.0f, sum);
+  return __builtin_fmaf(r, inv9, q);
+}
+
+// How mix(centre, avg, rate) (compute.wgsl:193-194) reduces for this rate:
+// rate 1 gives centre*0 + avg*1 = avg and rate 0 gives centre*1 + avg*0 =
+// centre exactly (centre and avg are finite and >= 0), so those two skip the
+// products (and rate 0 the window sum).
+enum MixMode { kMixGeneral = 0, kMixAverage = 1, kMixCentre = 2 };
+
+template <int M>
 __device__ inline float diffuse_out(float centre, float sum, const DevParams& p) {
-  float avg = sum / 9.0f;
+  if (M == kMixCentre) return centre;
+  float avg = div9(sum);
+  if (M == kMixAverage) return avg;
   float l = centre * p.one_minus_rate;
   float r = avg * p.rate;
   return l + r;
@@ -135,7 +173,7 @@ __device__ inline void finish_row(float (&t)[Codec<S>::kVec + 2], const RawRow<S
 }
 
 // One strip unit: 64*V columns x tile_rows rows of instance `in`.
-template <int S>
+template <int S, int M>
 __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Codec<S>::elem* out,
                                   const Instance& inst, const DevParams& p, uint32_t tile,
                                   uint32_t lane) {
@@ -171,20 +209,23 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
     if (more) issue_row<S>(ra, row_ptr(yafter), c, valid, need_left, need_right, lcol, rcol);
     finish_row<S>(next, rn, need_left, need_right, p);
     if (valid) {
-      uint32_t e[V];
+      float o[V];
 #pragma unroll
       for (int k = 0; k < V; k++) {
-        float s = prev[k] + prev[k + 1];
-        s = s + prev[k + 2];
-        s = s + cur[k];
-        s = s + cur[k + 1];
-        s = s + cur[k + 2];
-        s = s + next[k];
-        s = s + next[k + 1];
-        s = s + next[k + 2];
-        e[k] = Codec<S>::encode(diffuse_out(cur[k + 1], s, p));
+        float s = 0.0f;
+        if (M != kMixCentre) {
+          s = prev[k] + prev[k + 1];
+          s = s + prev[k + 2];
+          s = s + cur[k];
+          s = s + cur[k + 1];
+          s = s + cur[k + 2];
+          s = s + next[k];
+          s = s + next[k + 1];
+          s = s + next[k + 2];
+        }
+        o[k] = diffuse_out<M>(cur[k + 1], s, p);
       }
-      *reinterpret_cast<uint4*>(out + static_cast<size_t>(y) * w + c) = Codec<S>::pack(e);
+      *reinterpret_cast<uint4*>(out + static_cast<size_t>(y) * w + c) = Codec<S>::pack_values(o);
     }
 #pragma unroll
     for (int k = 0; k < V + 2; k++) {
@@ -217,7 +258,8 @@ __device__ inline void cell_unit(const typename Codec<S>::elem* in, typename Cod
       s = (dy == 0 && dx == 0) ? t : s + t;
     }
   }
-  out[i] = static_cast<typename Codec<S>::elem>(Codec<S>::encode(diffuse_out(centre, s, p)));
+  out[i] = static_cast<typename Codec<S>::elem>(
+      Codec<S>::encode(diffuse_out<kMixGeneral>(centre, s, p)));
 }
 
 struct TrailArgs {
@@ -245,10 +287,14 @@ __global__ __launch_bounds__(kTrailBlock) void trail_kernel(TrailArgs a) {
   const elem* in = static_cast<const elem*>(a.in) + inst.trail_off;
   elem* out = static_cast<elem*>(a.out) + inst.trail_off;
   uint32_t unit = u - inst.unit_off;
-  if (inst.ntx > 0)
-    strip_unit<S>(in, out, inst, p, unit, lane);
-  else
+  if (inst.ntx == 0)
     cell_unit<S>(in, out, inst, p, unit, lane);
+  else if (p.rate == 1.0f)
+    strip_unit<S, kMixAverage>(in, out, inst, p, unit, lane);
+  else if (p.rate == 0.0f)
+    strip_unit<S, kMixCentre>(in, out, inst, p, unit, lane);
+  else
+    strip_unit<S, kMixGeneral>(in, out, inst, p, unit, lane);
 }
 
 __global__ void set_params_kernel(DevParams* dst, DevParams src) { *dst = src; }

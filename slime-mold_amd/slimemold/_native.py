@@ -25,6 +25,7 @@ STORAGE_NAMES = {"f32": STORAGE_F32, "bf16": STORAGE_BF16, "f16": STORAGE_F16}
 
 FLAG_NO_XCD_REMAP = 0x1
 FLAG_F64_POSITIONS = 0x2
+FLAG_TILED_SENSING = 0x4
 
 
 class NativeLibraryMissing(RuntimeError):

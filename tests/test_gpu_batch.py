@@ -54,8 +54,10 @@ def test_batch_instances_equal_single_runs_and_oracle(slm, orc, storage):
 def test_batch_resort_and_graphs_do_not_change_results(slm):
     specs = mixed_specs(slm, seed=1)
     outs = []
-    for chunk, resort, cl in [(0, 0, 4), (10, 3, 4), (7, 2, 2)]:
-        with slm.Batch(specs, graph_chunk=chunk, resort_interval=resort, sort_cell_log2=cl) as b:
+    for chunk, resort, cl, tiled in [(0, 0, 4, False), (10, 3, 4, False), (7, 2, 2, True),
+                                     (5, 5, 5, True)]:
+        with slm.Batch(specs, graph_chunk=chunk, resort_interval=resort, sort_cell_log2=cl,
+                       tiled_sensing=tiled) as b:
             b.step(21)
             outs.append([(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))])
     for other in outs[1:]:
@@ -115,7 +117,8 @@ def test_f64_positions_match_oracle(slm, orc, name):
 
 def test_f64_positions_in_a_batch(slm):
     specs = mixed_specs(slm, seed=2)[:6]
-    with slm.Batch(specs, f64_positions=True, storage="f16") as b:
+    with slm.Batch(specs, f64_positions=True, storage="f16", resort_interval=3,
+                   tiled_sensing=True) as b:
         b.step(9)
         got = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
     for k, sp in enumerate(specs):

@@ -81,13 +81,13 @@ def test_zero_and_one_agent(slm, orc):
         assert_bitwise(gt, ot, f"n={n} trail")
 
 
-@pytest.mark.parametrize("interval", [1, 3])
-def test_resort_gives_identical_results(slm, orc, interval):
-    """Morton re-sort permutes agents internally; outputs are in original order
-    and the result must not change."""
+@pytest.mark.parametrize("interval,tiled", [(1, False), (3, False), (1, True), (4, True)])
+def test_resort_gives_identical_results(slm, orc, interval, tiled):
+    """Morton re-sort permutes agents internally and tiled sensing stages trail
+    windows in LDS; outputs are in original order and the result must not change."""
     s = preset(slm, "Firecracker Trees")
     ga, gt, oa, ot = run_parity(slm, orc, 512, 512, 1 << 16, 9, s, 20, resort_interval=interval,
-                                graph_chunk=0)
+                                graph_chunk=0, tiled_sensing=tiled)
     assert_bitwise(ga, oa, "agents")
     assert_bitwise(gt, ot, "trail")
 
@@ -95,9 +95,9 @@ def test_resort_gives_identical_results(slm, orc, interval):
 def test_graphs_match_eager(slm):
     s = preset(slm, "Snake")
     outs = []
-    for chunk, resort in [(0, 0), (10, 0), (10, 4), (7, 3)]:
+    for chunk, resort, tiled in [(0, 0, False), (10, 0, False), (10, 4, False), (7, 3, True)]:
         with slm.Simulation(384, 256, 30000, seed=4, params=s, graph_chunk=chunk,
-                            resort_interval=resort) as sim:
+                            resort_interval=resort, tiled_sensing=tiled) as sim:
             sim.step(33)
             sim.step(10)
             outs.append((sim.read_agents(), sim.read_trail()))
@@ -186,3 +186,25 @@ def test_rejects_bad_input_without_launching(slm):
         sim.write_trail(t)
         assert not np.signbit(sim.read_trail()).any()
         assert np.array_equal(sim.read_agents(), a)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_16bit_write_rounding_matches_oracle(slm, orc, storage):
+    """write_trail rounds to the storage format to nearest-even on the device;
+    the oracle's integer rounding (pinned against numpy/torch) must agree,
+    including subnormal results and exact ties."""
+    rng = np.random.default_rng(15)
+    h, w = 64, 96
+    vals = np.concatenate([rng.uniform(0, 1, 3000), rng.uniform(0, 1e-4, 1000),
+                           rng.uniform(0, 1e-37, 500), rng.uniform(0, 2, 500)])
+    # exact ties halfway between neighbours in both formats
+    ties = np.array([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -11, 0.5 + 2 ** -12],
+                    dtype=np.float32)
+    vals = np.concatenate([vals.astype(np.float32), ties])
+    t = np.zeros(h * w, np.float32)
+    t[:vals.size] = vals
+    t = t.reshape(h, w)
+    with slm.Simulation(w, h, 0, params=preset(slm, "Default"), storage=storage) as sim:
+        sim.write_trail(t)
+        got = sim.read_trail()
+    assert_bitwise(got, orc.round_storage(t, storage), f"{storage} rounding")
