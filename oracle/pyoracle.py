@@ -64,6 +64,8 @@ def lib():
             "orc_step_p64": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _D,
                                             _F, ctypes.POINTER(OrcParams), ctypes.c_int,
                                             ctypes.c_uint32, ctypes.c_int]),
+            "orc_trail_pass_f64": (None, [_D, _U8, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.POINTER(OrcParams), _D, ctypes.c_int]),
             "orc_total_mass": (ctypes.c_double, [_F, ctypes.c_uint64]),
             "orc_total_mass_f64": (ctypes.c_double, [_D, ctypes.c_uint64]),
         }
@@ -134,6 +136,29 @@ def trail_pass(trail: np.ndarray, p, mark=None, storage: str = "f32", nthreads: 
     lib().orc_trail_pass(_f(t), m, t.shape[1], t.shape[0], ctypes.byref(op), STORAGE[storage],
                          _f(out), nthreads)
     return out
+
+
+def trail_pass_f64(trail: np.ndarray, p, mark=None, nthreads: int = 0) -> np.ndarray:
+    """The trail pass in double precision (FP64 reference)."""
+    t = np.ascontiguousarray(trail, dtype=np.float64)
+    out = np.empty_like(t)
+    m = None
+    if mark is not None:
+        mk = np.ascontiguousarray(mark, dtype=np.uint8)
+        m = mk.ctypes.data_as(_U8)
+    lib().orc_trail_pass_f64(t.ctypes.data_as(_D), m, t.shape[1], t.shape[0],
+                             ctypes.byref(params(p)), out.ctypes.data_as(_D), nthreads)
+    return out
+
+
+def deposit_marks(agents: np.ndarray, w: int, h: int) -> np.ndarray:
+    """Cells the agents (positions after their move) deposit into (compute.wgsl:136-141)."""
+    x = agents[:, 0].astype(np.float64)
+    y = agents[:, 1].astype(np.float64)
+    ok = (x >= 0) & (x < w) & (y >= 0) & (y < h)
+    mark = np.zeros((h, w), np.uint8)
+    mark[np.floor(y[ok]).astype(np.int64), np.floor(x[ok]).astype(np.int64)] = 1
+    return mark
 
 
 def step(agents: np.ndarray, trail: np.ndarray, p, n_steps: int = 1, storage: str = "f32",

@@ -351,6 +351,36 @@ void orc_trail_pass(const float* tin, const uint8_t* mark, uint32_t w, uint32_t 
   free(t2);
 }
 
+/* The trail pass in double precision: the FP64 reference for the 16-bit
+ * storage modes (same deposit marks, decay, 3x3 toroidal box and mix). */
+void orc_trail_pass_f64(const double* tin, const uint8_t* mark, uint32_t w, uint32_t h,
+                        const orc_params* p, double* tout, int nthreads) {
+  uint64_t cells = (uint64_t)w * h;
+  double* t2 = (double*)malloc(cells * sizeof(double));
+  double k = (double)p->decay_factor * 0.001, dep = p->pheromone_deposition_amount;
+  double rate = p->diffusion_rate < 0.0f ? 0.0 : (p->diffusion_rate > 1.0f ? 1.0 : p->diffusion_rate);
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t c = 0; c < (int64_t)cells; c++) {
+    double v = tin[c];
+    if (mark && mark[c]) v = v + dep < 0.0 ? 0.0 : (v + dep > 1.0 ? 1.0 : v + dep);
+    v -= k;
+    t2[c] = v > 0.0 ? v : 0.0;
+  }
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t y = 0; y < (int64_t)h; y++) {
+    for (int64_t x = 0; x < (int64_t)w; x++) {
+      double sum = 0.0;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++)
+          sum += t2[((y + dy + h) % h) * w + (x + dx + w) % w];
+      double c = t2[y * w + x];
+      tout[y * w + x] = c * (1.0 - rate) + (sum / 9.0) * rate;
+    }
+  }
+  free(t2);
+}
+
 int orc_step(uint32_t w, uint32_t h, uint64_t n, float* ag, float* trail, const orc_params* p,
              int storage, uint32_t n_steps, int nthreads) {
   uint64_t cells = (uint64_t)w * h;

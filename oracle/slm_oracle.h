@@ -87,6 +87,10 @@ int64_t orc_agent_update(const float* trail, uint32_t w, uint32_t h, const orc_p
 void orc_trail_pass(const float* trail_in, const uint8_t* mark, uint32_t w, uint32_t h,
                     const orc_params* p, int storage, float* trail_out, int nthreads);
 
+/* The trail pass in double precision (FP64 reference for the storage modes). */
+void orc_trail_pass_f64(const double* trail_in, const uint8_t* mark, uint32_t w, uint32_t h,
+                        const orc_params* p, double* trail_out, int nthreads);
+
 /* n_steps full steps in place.  nthreads <= 0 uses all OpenMP threads. */
 int orc_step(uint32_t w, uint32_t h, uint64_t n, float* agents_aos, float* trail,
              const orc_params* p, int storage, uint32_t n_steps, int nthreads);

@@ -136,3 +136,76 @@ def test_no_device_is_reported_not_crashed(slm):
         pytest.skip("a GPU is visible")
     rc, msg, h = _create(slm)
     assert rc == -4 and not h.value
+
+
+CPP_PROBE = r"""
+#include <cstdio>
+#include "slimemold.hpp"
+int main() {
+  auto m = slime_mold::init_preset_manager();
+  std::printf("[");
+  bool first = true;
+  for (const auto& n : m.get_preset_names()) {
+    const auto& s = m.get_preset(n)->settings;
+    std::printf("%s{\"name\": \"%s\", \"settings\": {\"agent_count\": %zu, "
+                "\"agent_jitter\": %.9g, \"agent_sensor_angle\": %.9g, "
+                "\"agent_sensor_distance\": %.9g, \"agent_speed_max\": %.9g, "
+                "\"agent_speed_min\": %.9g, \"agent_turn_speed\": %.9g, \"blur_radius\": %.9g, "
+                "\"blur_sigma\": %.9g, \"pheromone_decay_factor\": %.9g, "
+                "\"pheromone_deposition_amount\": %.9g, \"pheromone_diffusion_rate\": %.9g, "
+                "\"window_height\": %u, \"window_width\": %u}}",
+                first ? "" : ", ", n.c_str(), s.agent_count, s.agent_jitter,
+                s.agent_sensor_angle, s.agent_sensor_distance, s.agent_speed_max,
+                s.agent_speed_min, s.agent_turn_speed, s.blur_radius, s.blur_sigma,
+                s.pheromone_decay_factor, s.pheromone_deposition_amount,
+                s.pheromone_diffusion_rate, s.window_height, s.window_width);
+    first = false;
+  }
+  std::printf("]\n");
+  if (m.get_preset("No such preset") != nullptr) return 3;
+  int code = 0;
+  try {  // with no device this must throw, never crash
+    slime_mold::Simulation sim(64, 64, 16, 1, slime_mold::Settings{});
+    sim.step(1);
+    sim.synchronize();
+  } catch (const slime_mold::Error& e) {
+    code = e.code;
+  }
+  std::fprintf(stderr, "create: %d\n", code);
+  return 0;
+}
+"""
+
+
+def test_cpp_mirror_compiles_and_matches_reference_presets(slm):
+    """slime-mold_amd/cpp/slimemold.hpp: C++17 header over the C ABI; its
+    preset table must equal the reference's (src/presets.rs:45-155)."""
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no g++")
+    import tempfile
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "presets.json")))["presets"]
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t.cpp")
+        with open(src, "w") as f:
+            f.write(CPP_PROBE)
+        libdir = os.path.join(ROOT, "slime-mold_amd", "slimemold")
+        subprocess.check_call([cxx, "-std=c++17", "-Wall", "-Wextra", "-Werror",
+                               "-I", os.path.join(ROOT, "slime-mold_amd", "cpp"), src, "-o",
+                               os.path.join(d, "t"), "-L", libdir, "-lslimemold_hip",
+                               f"-Wl,-rpath,{libdir}"])
+        r = subprocess.run([os.path.join(d, "t")], capture_output=True, text=True, check=True)
+    got = json.loads(r.stdout)
+    assert [p["name"] for p in got] == [p["name"] for p in ref]
+    f32 = lambda v: float(ctypes.c_float(v).value)  # noqa: E731
+    for g, e in zip(got, ref):
+        for k, v in e["settings"].items():
+            if isinstance(v, float):  # %.9g round-trips a float32 exactly
+                assert f32(g["settings"][k]) == f32(v), (g["name"], k)
+            else:
+                assert g["settings"][k] == v, (g["name"], k)
+    code = int(r.stderr.split("create:")[1])
+    if slm.device_count() == 0:
+        assert code == -4
+    else:
+        assert code == 0

@@ -286,3 +286,16 @@ def test_storage_rounding_modes(orc):
         out = orc.trail_pass(tt, params(), storage=st)
         f32out = orc.trail_pass(tt, params(), storage="f32")
         assert np.array_equal(out, orc.round_storage(f32out, st))
+
+
+def test_f64_trail_pass_matches_roll_convolution(orc):
+    rng = np.random.default_rng(16)
+    t = rng.uniform(0, 1, (24, 17))
+    mark = (rng.uniform(0, 1, t.shape) < 0.1).astype(np.uint8)
+    p = params(decay_factor=2.0, diffusion_rate=0.75, pheromone_deposition_amount=0.5)
+    got = orc.trail_pass_f64(t, p, mark)
+    k = float(np.float32(2.0)) * 0.001
+    t1 = np.where(mark == 1, np.clip(t + 0.5, 0.0, 1.0), t)
+    t2 = np.maximum(t1 - k, 0.0)
+    box = sum(np.roll(np.roll(t2, dy, 0), dx, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1))
+    assert np.abs(got - (t2 * 0.25 + box / 9.0 * 0.75)).max() <= 1e-15
