@@ -10,6 +10,11 @@ Batched leg: the 256-instance mix (grids 256^2..2048^2, 2^14..2^22 agents,
 parameters swept over the preset range) with a bf16 trail, all instances
 advanced by one agent launch and one trail launch per step.
 
+Before its warmup each leg lets the library's autotuner choose the launch
+tuning (re-sort interval and bin size, agents per workgroup, trail tile rows,
+tiled sensing) on its own state, untimed; the choice is reported under
+"tuning".
+
 Prints one JSON line: value = plain agent-updates/s over the timed region, the
 roofline of the plain leg's dominant kernel (per-kernel device time from HIP
 events on the library's stream, eager launches right after the timed region),
@@ -64,6 +69,9 @@ def parse():
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--sort-cell-log2", type=int, default=4, help="re-sort bin edge 2^k cells")
     ap.add_argument("--tiled", action="store_true", help="LDS-staged tiled sensing")
+    ap.add_argument("--autotune", type=int, default=20,
+                    help="autotune trial steps before each leg's warmup (0: keep the flags' "
+                         "tuning)")
     ap.add_argument("--timing-steps", type=int, default=30,
                     help="eager steps per preset for the per-kernel event timing")
     ap.add_argument("--batch-steps", type=int, default=1000, help="batched leg steps (0: skip)")
@@ -148,6 +156,7 @@ def plain_leg(slm, args):
     log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
         f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}")
     state0 = (sim.read_agents(), sim.read_trail()) if args.cpu_seconds > 0 else None
+    tuning = autotune(sim, args.autotune, "plain")
     sim.step(args.warmup)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -166,12 +175,24 @@ def plain_leg(slm, args):
         "agents_s": a_s, "trail_s": t_s, "sort_s": s_s, "mass": mass,
         "agents_roof": roofline("agents_kernel", a_bytes, a_s),
         "trail_roof": roofline("trail_kernel", t_bytes, t_s),
-        "cells": W * H, "agents": N, "state0": state0, "presets": presets,
+        "cells": W * H, "agents": N, "state0": state0, "presets": presets, "tuning": tuning,
     }
 
 
+def autotune(handle, trial_steps, label):
+    """Let the library pick the launch tuning (untimed; the state is kept)."""
+    if trial_steps <= 0:
+        return dict(handle.get_tuning().as_dict(), autotuned=False)
+    t0 = time.perf_counter()
+    t = handle.autotune(trial_steps)
+    d = t.as_dict()
+    d.update(autotuned=True, seconds=round(time.perf_counter() - t0, 2))
+    log(f"{label}: autotune {d}")
+    return d
+
+
 def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label,
-              tiled=False):
+              tiled=False, tune_steps=0):
     sb = STORAGE_BYTES[storage]
     log(f"{label}: {len(specs)} instances storage={storage} creating")
     b = slm.Batch(specs, storage=storage, resort_interval=resort, graph_chunk=graph_chunk,
@@ -179,6 +200,7 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
     info = b.info()
     log(f"{label}: {info.total_agents} agents, {info.total_cells} cells, "
         f"{info.device_bytes / 2**30:.1f} GiB; warmup {warmup}")
+    tuning = autotune(b, tune_steps, label)
     b.step(warmup)
     b.synchronize()
     t0 = time.perf_counter()
@@ -202,8 +224,7 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
         "diffuse_gbps": round(trail_bytes(sb) * n_cells / t_s / 1e9, 1),
         "diffuse_pct_hbm_peak": round(100 * trail_bytes(sb) * n_cells / t_s / 1e9 / HBM_PEAK_GBPS, 2),
         "agents_gbps": round(agent_bytes(sb) * n_agents / a_s / 1e9, 1),
-        "resort_interval": resort, "graph_chunk": graph_chunk, "sort_cell_log2": cell_log2,
-        "tiled_sensing": tiled, "total_mass_sum": float(masses.sum()),
+        "graph_chunk": graph_chunk, "tuning": tuning, "total_mass_sum": float(masses.sum()),
     }
 
 
@@ -226,7 +247,8 @@ def main():
     if args.workload == "combined":
         specs = slm.workloads.combined()
         res = batch_leg(slm, specs, "bf16", args.steps, args.warmup, args.resort,
-                        args.graph_chunk, args.sort_cell_log2, "combined", args.tiled)
+                        args.graph_chunk, args.sort_cell_log2, "combined", args.tiled,
+                        args.autotune)
         out = {
             "metric": METRIC, "value": res["agent_updates_per_s"], "unit": "agent-updates/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -262,8 +284,7 @@ def main():
                          f"hipGraph {args.graph_chunk}-step chunks"),
             "width": W, "height": H, "agents": N, "storage": args.storage, "seed": args.seed,
             "presets": slm.workloads.PLAIN_PRESETS, "graph_chunk": args.graph_chunk,
-            "resort_interval": args.resort, "sort_cell_log2": args.sort_cell_log2,
-            "tiled_sensing": args.tiled,
+            "tuning": plain["tuning"],
         },
         "roofline": dominant,
         "diffuse": {
@@ -289,7 +310,8 @@ def main():
     if args.batch_steps > 0:
         specs = slm.workloads.batch_mix(args.batch_count)
         bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.resort,
-                         args.graph_chunk, args.sort_cell_log2, "batched", args.tiled)
+                         args.graph_chunk, args.sort_cell_log2, "batched", args.tiled,
+                         args.autotune)
         out["batched"] = bres
         out["batched_vs_plain"] = round(bres["agent_updates_per_s"] / plain["value"], 3)
     if args.cpu_seconds > 0:

@@ -253,6 +253,50 @@ int slm_batch_write_trail(slm_batch* batch, uint32_t index, const float* trail, 
 int slm_batch_total_mass(slm_batch* batch, double* out, uint32_t count);
 int slm_batch_get_info(const slm_batch* batch, slm_batch_info* out);
 
+/* ---- launch tuning and the autotuner ---------------------------------------
+ * How the step is laid out on the GPU.  None of these settings changes a
+ * result bit; they only move time between kernels.  No counterpart in the
+ * reference, whose dispatch shape is fixed (agents: 64-wide workgroups,
+ * src/compute.wgsl:57 and src/main.rs:1168-1179; decay and diffuse: 16x16
+ * workgroups, src/compute.wgsl:148,164 and src/main.rs:1186-1233). */
+typedef struct slm_tuning {
+  uint32_t resort_interval;  /* Morton re-sort of agents every k steps; 0 = never */
+  uint32_t sort_cell_log2;   /* re-sort bin edge = 2^k cells, k <= 12 */
+  uint32_t agents_per_group; /* agents one workgroup of the agent pass updates:
+                                256, 512, 1024, 2048 or 4096 */
+  uint32_t trail_rows;       /* rows of one trail-pass tile (64*vector-width
+                                columns): 4, 8, 16, 32 or 64; 0 = chosen from the
+                                grid so the launch has >= 8192 waves */
+  uint32_t tiled_sensing;    /* 1 = SLM_FLAG_TILED_SENSING (needs re-sorting) */
+  uint32_t trials;           /* autotune: configurations timed; else 0 */
+  uint32_t reserved[2];      /* must be zero */
+  /* autotune, for the chosen tuning: step_ms = device time per step from the
+     first launch to the end of the last (eager launches, gaps included);
+     agents/trail/sort_ms = per-step sums of those kernels' own times (sorts
+     amortised over their interval); else 0 */
+  double step_ms, agents_ms, trail_ms, sort_ms;
+} slm_tuning;
+
+int slm_get_tuning(const slm_sim* sim, slm_tuning* out);
+/* Applies a tuning.  Agent and trail state, the step count and the parameters
+ * are kept; captured graphs are rebuilt on the next slm_step. */
+int slm_set_tuning(slm_sim* sim, const slm_tuning* tuning);
+/* Times candidate tunings on the simulation's own state and keeps the
+ * fastest: the trail tile rows by trail-pass time, then the re-sort interval,
+ * the bin size, tiled sensing and agents per workgroup by total step time
+ * (coordinate descent; every trial runs at least `trial_steps` steps, whole
+ * re-sort intervals, from a copy of the current state).  The state the call
+ * found is restored afterwards, so stepping on gives the same bits as without
+ * the call.  out (may be NULL) receives the chosen tuning and its timing.
+ * Needs device memory for one extra copy of the agents and the trail, plus the
+ * re-sort buffers when re-sorting was off; candidates that do not fit are
+ * skipped. */
+int slm_autotune(slm_sim* sim, uint32_t trial_steps, slm_tuning* out);
+
+int slm_batch_get_tuning(const slm_batch* batch, slm_tuning* out);
+int slm_batch_set_tuning(slm_batch* batch, const slm_tuning* tuning);
+int slm_batch_autotune(slm_batch* batch, uint32_t trial_steps, slm_tuning* out);
+
 #ifdef __cplusplus
 }
 #endif

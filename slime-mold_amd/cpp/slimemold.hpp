@@ -220,6 +220,17 @@ class Simulation {
     check(slm_total_mass(h_, &m));
     return m;
   }
+  slm_tuning tuning() const {
+    slm_tuning t;
+    check(slm_get_tuning(h_, &t));
+    return t;
+  }
+  void set_tuning(const slm_tuning& t) { check(slm_set_tuning(h_, &t)); }
+  slm_tuning autotune(uint32_t trial_steps = 20) {
+    slm_tuning t;
+    check(slm_autotune(h_, trial_steps, &t));
+    return t;
+  }
   slm_sim* handle() { return h_; }
 
  private:
@@ -276,6 +287,17 @@ class Batch {
     std::vector<float> a(inst_.at(k).agent_count * 4);
     check(slm_batch_read_agents(h_, k, a.data(), a.size() / 4));
     return a;
+  }
+  slm_tuning tuning() const {
+    slm_tuning t;
+    check(slm_batch_get_tuning(h_, &t));
+    return t;
+  }
+  void set_tuning(const slm_tuning& t) { check(slm_batch_set_tuning(h_, &t)); }
+  slm_tuning autotune(uint32_t trial_steps = 20) {
+    slm_tuning t;
+    check(slm_batch_autotune(h_, trial_steps, &t));
+    return t;
   }
   std::vector<double> total_mass() {
     std::vector<double> m(inst_.size());

@@ -390,28 +390,47 @@ struct AgentArgs {
   void* trail;
   const Instance* inst;
   const DevParams* params;
-  const uint32_t* block_inst;
+  const uint32_t* block_inst;  // per kAgentBlock slots
   uint32_t count, total;
+  uint32_t group;  // agents per workgroup, a multiple of kAgentBlock
   int xcd_remap;
 };
 
 template <int S, bool P64>
 __global__ __launch_bounds__(kAgentBlock) void agents_kernel(AgentArgs args) {
   uint32_t blk = args.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  uint32_t first = blk * kAgentBlock;
-  uint32_t last = min(first + kAgentBlock, args.total) - 1u;
-  uint32_t j = first + threadIdx.x;
   using elem = typename TrailRead<S>::elem;
-  // instances overlapping this block, one at a time (block-uniform)
-  for (uint32_t ii = args.block_inst[blk];; ii++) {
+  if (args.group == kAgentBlock) {  // one agent per thread
+    uint32_t first = blk * kAgentBlock;
+    uint32_t last = min(first + kAgentBlock, args.total) - 1u;
+    uint32_t j = first + threadIdx.x;
+    // instances overlapping this block, one at a time (block-uniform)
+    for (uint32_t ii = args.block_inst[blk];; ii++) {
+      const Instance in = args.inst[ii];
+      uint32_t end = in.agent_off + in.agent_count;
+      if (j >= in.agent_off && j < end) {
+        const DevParams p = args.params[ii];
+        GlobalTaps<S> taps{static_cast<elem*>(args.trail) + in.trail_off, in.width, in.height};
+        agent_step<P64>(args.a, j, in, p, taps);
+      }
+      if (last < end || ii + 1u >= args.count) break;
+    }
+    return;
+  }
+  // `group` consecutive slots, each thread striding by the block size with the
+  // next agent's state in flight
+  uint32_t first = blk * args.group;
+  uint32_t stop = min(first + args.group, args.total);
+  for (uint32_t ii = args.block_inst[first / kAgentBlock];; ii++) {
     const Instance in = args.inst[ii];
     uint32_t end = in.agent_off + in.agent_count;
-    if (j >= in.agent_off && j < end) {
+    uint32_t b = max(first, in.agent_off), e = min(stop, end);
+    if (b < e) {
       const DevParams p = args.params[ii];
       GlobalTaps<S> taps{static_cast<elem*>(args.trail) + in.trail_off, in.width, in.height};
-      agent_step<P64>(args.a, j, in, p, taps);
+      agent_loop<P64, kAgentBlock>(args.a, b, e, in, p, taps);
     }
-    if (last < end || ii + 1u >= args.count) break;
+    if (stop <= end || ii + 1u >= args.count) break;
   }
 }
 
@@ -607,7 +626,7 @@ __global__ void unpack_agents_kernel(AgentSoA a, uint32_t off, uint32_t n, int p
 
 template <int S>
 void launch_agents_t(const Pool& pool, const AgentArgs& args, hipStream_t stream) {
-  uint32_t nb = blocks_for(pool.total_agents, kAgentBlock);
+  uint32_t nb = blocks_for(pool.total_agents, args.group);
   if (pool.pos64)
     agents_kernel<S, true><<<nb, kAgentBlock, 0, stream>>>(args);
   else
@@ -618,8 +637,9 @@ void launch_agents_t(const Pool& pool, const AgentArgs& args, hipStream_t stream
 
 void launch_agents(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t stream) {
   if (pool.total_agents == 0) return;
-  AgentArgs args{a, trail, pool.inst, pool.params, pool.agent_block_inst, pool.count,
-                 pool.total_agents, pool.xcd_remap};
+  uint32_t group = pool.agents_per_group ? pool.agents_per_group : kAgentBlock;
+  AgentArgs args{a,          trail,      pool.inst,         pool.params, pool.agent_block_inst,
+                 pool.count, pool.total_agents, group, pool.xcd_remap};
   switch (pool.storage) {
     case SLM_STORAGE_BF16:
       launch_agents_t<SLM_STORAGE_BF16>(pool, args, stream);

@@ -108,7 +108,23 @@ struct EngineConfig {
   uint32_t graph_chunk = 0;
   uint32_t sort_cell_log2 = 4;
   uint32_t flags = 0;
+  uint32_t agents_per_group = kAgentBlock;
+  uint32_t trail_rows = 0;  // 0: chosen per instance by plan_trail
 };
+
+int check_tuning(const slm_tuning& t) {
+  if (t.sort_cell_log2 > 12) return fail(SLM_ERR_INVALID_ARGUMENT, "sort_cell_log2 must be <= 12");
+  uint32_t g = t.agents_per_group;
+  if (g < kAgentBlock || g > 16u * kAgentBlock || (g & (g - 1u)))
+    return fail(SLM_ERR_INVALID_ARGUMENT, "agents_per_group must be 256, 512, 1024, 2048 or 4096");
+  uint32_t r = t.trail_rows;
+  if (r != 0 && (r < 4 || r > 64 || (r & (r - 1u))))
+    return fail(SLM_ERR_INVALID_ARGUMENT, "trail_rows must be 0, 4, 8, 16, 32 or 64");
+  if (t.tiled_sensing > 1) return fail(SLM_ERR_INVALID_ARGUMENT, "tiled_sensing must be 0 or 1");
+  if (t.reserved[0] || t.reserved[1])
+    return fail(SLM_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
+  return SLM_OK;
+}
 
 int check_engine_config(const EngineConfig& c, const uint32_t* reserved, size_t nreserved) {
   if (c.storage < SLM_STORAGE_F32 || c.storage > SLM_STORAGE_F16)
@@ -178,6 +194,7 @@ class Engine {
     p.item_total = item_total_d_;
     p.max_items = max_items_;
     p.resort_interval = cfg.resort_interval;
+    p.agents_per_group = cfg.agents_per_group;
     return p;
   }
 
@@ -198,46 +215,68 @@ class Engine {
     return SLM_OK;
   }
 
+  // Steps with HIP events around every kernel.  Events for up to kTimedBatch
+  // steps are recorded before the host waits, so the stream never drains
+  // between steps and host submission gaps stay out of the kernel times.
   int step_timed(uint32_t n, slm_timing* out) {
     std::memset(out, 0, sizeof *out);
-    hipEvent_t ev[5];
-    for (auto& e : ev) SLM_HIP(hipEventCreate(&e));
+    constexpr uint32_t kTimedBatch = 128;
+    constexpr int kEv = 5;  // sort begin/end, agents begin, agents end = trail begin, trail end
+    std::vector<hipEvent_t> ev;
     int rc = SLM_OK;
+    uint32_t batch = n < kTimedBatch ? n : kTimedBatch;
+    for (uint32_t i = 0; i < batch * kEv && rc == SLM_OK; i++) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess)
+        rc = fail(SLM_ERR_HIP, "hipEventCreate failed");
+      else
+        ev.push_back(e);
+    }
+    std::vector<uint8_t> sorted(batch);
     Pool p = pool();
-    for (uint32_t i = 0; i < n && rc == SLM_OK; i++) {
-      bool sort = sort_due(steps_done) && total_agents_ > 0;
-      if (sort) {
-        (void)hipEventRecord(ev[0], stream);
-        launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
-        (void)hipEventRecord(ev[1], stream);
-        acur_ ^= 1;
+    for (uint32_t done = 0; done < n && rc == SLM_OK;) {
+      uint32_t m = n - done < batch ? n - done : batch;
+      for (uint32_t i = 0; i < m; i++) {
+        hipEvent_t* e = &ev[static_cast<size_t>(i) * kEv];
+        bool sort = sort_due(steps_done) && total_agents_ > 0;
+        sorted[i] = sort;
+        if (sort) {
+          (void)hipEventRecord(e[0], stream);
+          launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
+          (void)hipEventRecord(e[1], stream);
+          acur_ ^= 1;
+          tiles_ready_ = true;
+        }
+        (void)hipEventRecord(e[2], stream);
+        enqueue_agents(p);
+        (void)hipEventRecord(e[3], stream);
+        launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
+        (void)hipEventRecord(e[4], stream);
+        cur_ ^= 1;
+        steps_done++;
       }
-      if (sort) tiles_ready_ = true;
-      (void)hipEventRecord(ev[2], stream);
-      enqueue_agents(p);
-      (void)hipEventRecord(ev[3], stream);
-      launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
-      (void)hipEventRecord(ev[4], stream);
-      cur_ ^= 1;
-      steps_done++;
-      hipError_t e = hipEventSynchronize(ev[4]);
-      if (e != hipSuccess) {
-        rc = fail(SLM_ERR_HIP, "step failed: %s", hipGetErrorString(e));
+      hipError_t err = hipEventSynchronize(ev[static_cast<size_t>(m - 1) * kEv + 4]);
+      if (err != hipSuccess) {
+        rc = fail(SLM_ERR_HIP, "step failed: %s", hipGetErrorString(err));
         break;
       }
-      float ms = 0.0f;
-      if (sort && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
-        out->sort_ms += ms;
-        out->sort_launches++;
+      for (uint32_t i = 0; i < m; i++) {
+        hipEvent_t* e = &ev[static_cast<size_t>(i) * kEv];
+        float ms = 0.0f;
+        if (sorted[i] && hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) {
+          out->sort_ms += ms;
+          out->sort_launches++;
+        }
+        if (total_agents_ > 0 && hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) {
+          out->agents_ms += ms;
+          out->agents_launches++;
+        }
+        if (hipEventElapsedTime(&ms, e[3], e[4]) == hipSuccess) {
+          out->trail_ms += ms;
+          out->trail_launches++;
+        }
       }
-      if (total_agents_ > 0 && hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) {
-        out->agents_ms += ms;
-        out->agents_launches++;
-      }
-      if (hipEventElapsedTime(&ms, ev[3], ev[4]) == hipSuccess) {
-        out->trail_ms += ms;
-        out->trail_launches++;
-      }
+      done += m;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
@@ -387,7 +426,276 @@ class Engine {
     return sync_checked("agent re-initialisation");
   }
 
+  // ---- launch tuning ----------------------------------------------------------
+  slm_tuning tuning() const {
+    slm_tuning t;
+    std::memset(&t, 0, sizeof t);
+    t.resort_interval = cfg.resort_interval;
+    t.sort_cell_log2 = cfg.sort_cell_log2;
+    t.agents_per_group = cfg.agents_per_group;
+    t.trail_rows = cfg.trail_rows;
+    t.tiled_sensing = (cfg.flags & SLM_FLAG_TILED_SENSING) ? 1u : 0u;
+    return t;
+  }
+
+  // Keeps agents, trail, step count and parameters.
+  int set_tuning(const slm_tuning& t) {
+    SLM_TRY(sync());
+    if (!fits(t)) return fail(SLM_ERR_OUT_OF_MEMORY, "not enough device memory for this tuning");
+    if (sorting() == (t.resort_interval > 0)) return apply_tuning(t);
+    Snapshot snap;
+    int rc = take_snapshot(snap, false);
+    if (rc == SLM_OK) rc = apply_tuning(t);
+    if (rc == SLM_OK) rc = restore_snapshot(snap);
+    if (rc == SLM_OK) rc = sync_checked("re-tuning");
+    free_snapshot(snap);
+    return rc;
+  }
+
+  // Coordinate descent over the tuning fields, each candidate timed on a copy
+  // of the current state; the state is put back afterwards.
+  int autotune(uint32_t trial_steps, slm_tuning* out) {
+    SLM_TRY(sync());
+    Snapshot snap;
+    int rc = take_snapshot(snap, true);
+    if (rc) {
+      free_snapshot(snap);
+      return rc;
+    }
+    const slm_tuning start = tuning();
+    const uint64_t steps0 = steps_done;
+    slm_tuning best = start;
+    Trial bt{};
+    trials_ = 0;
+    rc = search(snap, trial_steps ? trial_steps : 1u, best, bt);
+    int rc2 = put_tuning(rc == SLM_OK ? best : start, snap);
+    steps_done = steps0;
+    free_snapshot(snap);
+    if (rc) return rc;
+    SLM_TRY(rc2);
+    if (out) {
+      *out = tuning();
+      out->trials = trials_;
+      out->step_ms = bt.step;
+      out->agents_ms = bt.agents;
+      out->trail_ms = bt.trail;
+      out->sort_ms = bt.sort;
+    }
+    return SLM_OK;
+  }
+
  private:
+  // Agents in original order (and optionally the current trail), in
+  // short-lived buffers.
+  struct Snapshot {
+    AgentSoA a{};
+    void* trail = nullptr;
+  };
+  struct Trial {
+    double step, agents, trail, sort;  // ms per step, sorts amortised
+    bool ok;
+  };
+  uint32_t trials_ = 0;
+
+  int take_snapshot(Snapshot& s, bool with_trail) {
+    uint64_t n = total_agents_;
+    if (pos64()) {
+      SLM_TRY(scratch_alloc(reinterpret_cast<void**>(&s.a.x64), n * 8));
+      SLM_TRY(scratch_alloc(reinterpret_cast<void**>(&s.a.y64), n * 8));
+    } else {
+      SLM_TRY(scratch_alloc(reinterpret_cast<void**>(&s.a.x), n * 4));
+      SLM_TRY(scratch_alloc(reinterpret_cast<void**>(&s.a.y), n * 4));
+    }
+    SLM_TRY(scratch_alloc(reinterpret_cast<void**>(&s.a.th), n * 4));
+    SLM_TRY(scratch_alloc(reinterpret_cast<void**>(&s.a.v), n * 4));
+    launch_canonical_copy(pool(), agents_[acur_], s.a, stream);
+    if (with_trail) {
+      size_t bytes = trail_elems_ * storage_bytes(cfg.storage);
+      SLM_TRY(scratch_alloc(&s.trail, bytes));
+      SLM_HIP(hipMemcpyAsync(s.trail, trail_[cur_], bytes, hipMemcpyDeviceToDevice, stream));
+    }
+    return sync_checked("snapshot");
+  }
+
+  int restore_snapshot(const Snapshot& s) {
+    launch_canonical_copy(pool(), s.a, agents_[acur_], stream);
+    if (s.trail)
+      SLM_HIP(hipMemcpyAsync(trail_[cur_], s.trail, trail_elems_ * storage_bytes(cfg.storage),
+                             hipMemcpyDeviceToDevice, stream));
+    SLM_HIP(hipGetLastError());
+    return SLM_OK;
+  }
+
+  void free_snapshot(Snapshot& s) {
+    for (void* p : {static_cast<void*>(s.a.x), static_cast<void*>(s.a.y),
+                    static_cast<void*>(s.a.x64), static_cast<void*>(s.a.y64),
+                    static_cast<void*>(s.a.th), static_cast<void*>(s.a.v), s.trail})
+      if (p) (void)hipFree(p);
+    s = Snapshot{};
+  }
+
+  uint64_t bins_for(uint32_t cell_log2) const {
+    uint64_t bins = 0;
+    for (const Instance& i : inst) {
+      Instance c = i;
+      plan_bins(c, cell_log2);
+      bins += c.nbins;
+    }
+    return bins;
+  }
+
+  // Whether switching to `t` fits the free device memory (counting the
+  // snapshot an agent re-layout needs and not counting what it frees).
+  bool fits(const slm_tuning& t) {
+    bool sort = t.resort_interval > 0;
+    uint64_t need = 64ull << 20;
+    if (sort) {
+      uint64_t bins = bins_for(t.sort_cell_log2);
+      if (bins > kMaxTotalBins) return false;
+      need += bins * 12ull;
+    }
+    if (sort != sorting()) {
+      uint64_t n = total_agents_;
+      uint64_t pos = pos64() ? 16 : 8;
+      need += n * ((pos + 8 + (sort ? 4 : 0)) * (sort ? 2 : 1) + (sort ? 8 : 0));  // new arrays
+      need += n * (pos + 8);                                                      // snapshot
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    return need <= free_b;
+  }
+
+  // Sets the tuning fields and rebuilds what depends on them.  Switching
+  // re-sorting on or off reallocates the agent arrays, losing their contents:
+  // the caller restores them.
+  int apply_tuning(const slm_tuning& t) {
+    bool relayout = sorting() != (t.resort_interval > 0);
+    drop_graphs();
+    if (relayout) free_agent_layout();
+    cfg.resort_interval = t.resort_interval;
+    cfg.sort_cell_log2 = t.sort_cell_log2;
+    cfg.agents_per_group = t.agents_per_group;
+    cfg.trail_rows = t.trail_rows;
+    if (t.tiled_sensing)
+      cfg.flags |= SLM_FLAG_TILED_SENSING;
+    else
+      cfg.flags &= ~SLM_FLAG_TILED_SENSING;
+    if (relayout) SLM_TRY(layout_agents());
+    return layout_trail_tables();
+  }
+
+  int put_tuning(const slm_tuning& t, const Snapshot& snap) {
+    SLM_TRY(apply_tuning(t));
+    SLM_TRY(restore_snapshot(snap));
+    return sync_checked("autotune restore");
+  }
+
+  // Times `t` over at least `steps` steps (whole re-sort intervals) from the
+  // snapshot; r.ok is false when it does not fit in memory.
+  int trial(const slm_tuning& t, const Snapshot& snap, uint32_t steps, Trial& r) {
+    r = Trial{};
+    if (!fits(t)) return SLM_OK;
+    SLM_TRY(put_tuning(t, snap));
+    // Warm up for one re-sort interval (at least two steps) so the timed
+    // steps start where a long run would be: the first timed step re-sorts
+    // agents that have drifted for a whole interval.
+    uint32_t k = t.resort_interval;
+    uint32_t warm = k > 2 ? k : 2;
+    steps_done = 0;
+    for (uint32_t i = 0; i < warm; i++) enqueue_step(sort_due(steps_done + i));
+    steps_done = k ? k : warm;
+    uint32_t n = steps;
+    if (k) n = (steps + k - 1) / k * k;
+    // the step time is the span from the first launch to the end of the last,
+    // gaps included; the kernel times come from the events around each kernel
+    hipEvent_t span[2] = {nullptr, nullptr};
+    SLM_HIP(hipEventCreate(&span[0]));
+    if (hipEventCreate(&span[1]) != hipSuccess) {
+      (void)hipEventDestroy(span[0]);
+      return fail(SLM_ERR_HIP, "hipEventCreate failed");
+    }
+    slm_timing tm;
+    (void)hipEventRecord(span[0], stream);
+    int rc = step_timed(n, &tm);
+    (void)hipEventRecord(span[1], stream);
+    float ms = 0.0f;
+    if (rc == SLM_OK) {
+      hipError_t e = hipEventSynchronize(span[1]);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, span[0], span[1]);
+      if (e != hipSuccess) rc = fail(SLM_ERR_HIP, "trial timing failed: %s", hipGetErrorString(e));
+    }
+    (void)hipEventDestroy(span[0]);
+    (void)hipEventDestroy(span[1]);
+    SLM_TRY(rc);
+    r.agents = tm.agents_ms / n;
+    r.trail = tm.trail_ms / n;
+    r.sort = tm.sort_ms / n;
+    r.step = static_cast<double>(ms) / n;
+    r.ok = true;
+    trials_++;
+    return SLM_OK;
+  }
+
+  int search(const Snapshot& snap, uint32_t steps, slm_tuning& best, Trial& bt) {
+    Trial r;
+    SLM_TRY(trial(best, snap, steps, r));  // warm-up: clocks, code objects, caches
+    SLM_TRY(trial(best, snap, steps, bt));
+    if (!bt.ok) return fail(SLM_ERR_OUT_OF_MEMORY, "autotune: no memory for trials");
+    const double kGain = 0.995;  // a change must save at least 0.5%
+    auto take = [&](const slm_tuning& t) -> int {
+      SLM_TRY(trial(t, snap, steps, r));
+      if (r.ok && r.step < bt.step * kGain) {
+        best = t;
+        bt = r;
+      }
+      return SLM_OK;
+    };
+    // 1. trail tile rows, judged by the trail pass alone
+    {
+      const slm_tuning base = best;
+      double cur = bt.trail;
+      for (uint32_t rows : {4u, 8u, 16u, 32u, 64u}) {
+        slm_tuning t = base;
+        t.trail_rows = rows;
+        SLM_TRY(trial(t, snap, steps, r));
+        if (r.ok && r.trail < cur * kGain) {
+          cur = r.trail;
+          best = t;
+          bt = r;
+        }
+      }
+    }
+    // 2. re-sort interval, 3. bin size, 4. tiled sensing, 5. agents per group
+    for (uint32_t k : {0u, 5u, 10u, 20u, 40u, 80u}) {
+      if (k == best.resort_interval) continue;
+      slm_tuning t = best;
+      t.resort_interval = k;
+      SLM_TRY(take(t));
+    }
+    if (best.resort_interval) {
+      const uint32_t cl = best.sort_cell_log2;
+      for (uint32_t c : {cl - 1u, cl + 1u}) {
+        if (cl == 0 || c < 2 || c > 8) continue;
+        slm_tuning t = best;
+        t.sort_cell_log2 = c;
+        SLM_TRY(take(t));
+      }
+      slm_tuning t = best;
+      t.tiled_sensing ^= 1u;
+      SLM_TRY(take(t));
+    }
+    for (uint32_t g : {256u, 512u, 1024u, 2048u, 4096u}) {
+      if (g == best.agents_per_group) continue;
+      slm_tuning t = best;
+      t.agents_per_group = g;
+      SLM_TRY(take(t));
+    }
+    return SLM_OK;
+  }
+
   Instance* inst_d_ = nullptr;
   DevParams* params_d_ = nullptr;
   uint32_t* agent_block_d_ = nullptr;
@@ -583,7 +891,7 @@ class Engine {
       Instance& in = inst[k];
       uint32_t target = static_cast<uint32_t>(
           std::max<uint64_t>(1, kTargetWaves * cells(k) / std::max<uint64_t>(1, total_cells)));
-      plan_trail(in, cfg.storage, target);
+      plan_trail(in, cfg.storage, target, cfg.trail_rows);
       in.unit_off = static_cast<uint32_t>(units);
       units += in.units;
       uint32_t cl = cfg.sort_cell_log2;
@@ -967,6 +1275,34 @@ int slm_get_info(const slm_sim* s, slm_info* out) {
   return SLM_OK;
 }
 
+int slm_get_tuning(const slm_sim* s, slm_tuning* out) {
+  if (!s || !s->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = s->eng->tuning();
+  return SLM_OK;
+}
+
+static void sync_config(slm_config& c, const Engine& e) {
+  c.resort_interval = e.cfg.resort_interval;
+  c.sort_cell_log2 = e.cfg.sort_cell_log2;
+  c.flags = e.cfg.flags;
+}
+
+int slm_set_tuning(slm_sim* s, const slm_tuning* t) {
+  SIM_GUARD(s);
+  if (!t) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_tuning(*t));
+  int rc = s->eng->set_tuning(*t);
+  sync_config(s->cfg, *s->eng);
+  return rc;
+}
+
+int slm_autotune(slm_sim* s, uint32_t trial_steps, slm_tuning* out) {
+  SIM_GUARD(s);
+  int rc = s->eng->autotune(trial_steps, out);
+  sync_config(s->cfg, *s->eng);
+  return rc;
+}
+
 // ---- batches ------------------------------------------------------------------
 
 int slm_batch_config_default(slm_batch_config* cfg) {
@@ -1100,6 +1436,34 @@ int slm_batch_total_mass(slm_batch* b, double* out, uint32_t count) {
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(count, b->eng->count(), "total_mass (count != instances)"));
   return b->eng->total_mass(out);
+}
+
+static void sync_batch_config(slm_batch_config& c, const Engine& e) {
+  c.resort_interval = e.cfg.resort_interval;
+  c.sort_cell_log2 = e.cfg.sort_cell_log2;
+  c.flags = e.cfg.flags;
+}
+
+int slm_batch_get_tuning(const slm_batch* b, slm_tuning* out) {
+  if (!b || !b->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = b->eng->tuning();
+  return SLM_OK;
+}
+
+int slm_batch_set_tuning(slm_batch* b, const slm_tuning* t) {
+  BATCH_GUARD(b);
+  if (!t) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_tuning(*t));
+  int rc = b->eng->set_tuning(*t);
+  sync_batch_config(b->cfg, *b->eng);
+  return rc;
+}
+
+int slm_batch_autotune(slm_batch* b, uint32_t trial_steps, slm_tuning* out) {
+  BATCH_GUARD(b);
+  int rc = b->eng->autotune(trial_steps, out);
+  sync_batch_config(b->cfg, *b->eng);
+  return rc;
 }
 
 int slm_batch_get_info(const slm_batch* b, slm_batch_info* out) {

@@ -89,6 +89,7 @@ struct Pool {
   uint32_t* item_total;        // device: number of valid work items
   uint32_t max_items;
   uint32_t resort_interval;    // steps agents may drift from their tile between re-sorts
+  uint32_t agents_per_group;   // agents per workgroup of the global agent pass (k * kAgentBlock)
 };
 
 constexpr uint32_t kTileLog2 = 6;  // sensing tiles of 64 x 64 cells
@@ -101,7 +102,9 @@ inline uint64_t storage_bytes(int storage) { return storage == SLM_STORAGE_F32 ?
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
 
 // Host-side layout of one instance's trail pass: fills tile_rows, ntx, units.
-void plan_trail(Instance& in, int storage, uint32_t target_waves);
+// rows > 0 fixes the tile height; 0 takes the tallest of 64..4 rows that still
+// gives target_waves units.
+void plan_trail(Instance& in, int storage, uint32_t target_waves, uint32_t rows);
 // Host-side layout of one instance's re-sort bins: fills nbins, nbx, nby.
 void plan_bins(Instance& in, uint32_t cell_log2);
 // Tiles of 2^tile_log2 cells (>= the bin edge) for tiled sensing: fills
@@ -154,5 +157,10 @@ struct SortScratch {
 uint32_t plan_scan(SortScratch& s, uint32_t nbins);
 void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& dst,
                         const SortScratch& s, hipStream_t stream);
+// Copies every agent of the pool from src into dst in original order (slot
+// agent_off + id, or the same slot when src has no id table); dst.id, if
+// present, receives the identity.  src and dst must not overlap.
+void launch_canonical_copy(const Pool& pool, const AgentSoA& src, const AgentSoA& dst,
+                           hipStream_t stream);
 
 }  // namespace slm

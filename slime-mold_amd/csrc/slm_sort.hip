@@ -175,6 +175,28 @@ __global__ void scatter_kernel(AgentSoA src, AgentSoA dst, uint32_t n, int pos64
   dst.id[d] = src.id[j];
 }
 
+__global__ __launch_bounds__(kAgentBlock) void canonical_copy_kernel(
+    AgentSoA src, AgentSoA dst, const Instance* insts, const uint32_t* block_inst, uint32_t total,
+    int pos64) {
+  uint32_t j = blockIdx.x * kAgentBlock + threadIdx.x;
+  if (j >= total) return;
+  uint32_t ii = block_inst[blockIdx.x];
+  while (j >= insts[ii].agent_off + insts[ii].agent_count) ii++;  // ends: j < total
+  uint32_t off = insts[ii].agent_off;
+  uint32_t local = src.id ? src.id[j] : j - off;
+  uint32_t d = off + local;
+  if (pos64) {
+    dst.x64[d] = src.x64[j];
+    dst.y64[d] = src.y64[j];
+  } else {
+    dst.x[d] = src.x[j];
+    dst.y[d] = src.y[j];
+  }
+  dst.th[d] = src.th[j];
+  dst.v[d] = src.v[j];
+  if (dst.id) dst.id[d] = local;
+}
+
 // Work items of each tile: ceil(agents / kItemAgents), 0 for an empty tile.
 __global__ void tile_count_kernel(const Instance* insts, const uint32_t* tile_inst,
                                   uint32_t ntiles, uint32_t cell_log2, const uint32_t* offset,
@@ -277,6 +299,13 @@ void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& d
         pool.inst, pool.tile_inst, nt, pool.sort_cell_log2, s.offset, s.tile_items, pool.items,
         pool.item_total);
   }
+}
+
+void launch_canonical_copy(const Pool& pool, const AgentSoA& src, const AgentSoA& dst,
+                           hipStream_t stream) {
+  if (pool.total_agents == 0) return;
+  canonical_copy_kernel<<<blocks_for(pool.total_agents, kAgentBlock), kAgentBlock, 0, stream>>>(
+      src, dst, pool.inst, pool.agent_block_inst, pool.total_agents, pool.pos64);
 }
 
 }  // namespace slm

@@ -361,13 +361,15 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
 
 }  // namespace
 
-void plan_trail(Instance& in, int storage, uint32_t target_waves) {
+void plan_trail(Instance& in, int storage, uint32_t target_waves, uint32_t fixed_rows) {
   uint32_t v = storage == SLM_STORAGE_F32 ? 4u : 8u;
   if (in.width % v == 0) {
     in.ntx = (in.width + 64u * v - 1u) / (64u * v);
-    // rows per strip unit: the longest of 64..4 that still gives target_waves units
-    uint32_t rows = 64;
-    while (rows > 4 && static_cast<uint64_t>(in.ntx) * ((in.height + rows - 1) / rows) < target_waves)
+    // rows per strip unit: fixed, or the longest of 64..4 that still gives
+    // target_waves units
+    uint32_t rows = fixed_rows ? fixed_rows : 64u;
+    while (!fixed_rows && rows > 4 &&
+           static_cast<uint64_t>(in.ntx) * ((in.height + rows - 1) / rows) < target_waves)
       rows >>= 1;
     in.tile_rows = rows;
     in.units = in.ntx * ((in.height + rows - 1u) / rows);

@@ -134,6 +134,26 @@ class BatchInfo(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """slm_tuning: launch layout of the step (never changes results)."""
+    _fields_ = [
+        ("resort_interval", ctypes.c_uint32),
+        ("sort_cell_log2", ctypes.c_uint32),
+        ("agents_per_group", ctypes.c_uint32),
+        ("trail_rows", ctypes.c_uint32),
+        ("tiled_sensing", ctypes.c_uint32),
+        ("trials", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
+        ("step_ms", ctypes.c_double),
+        ("agents_ms", ctypes.c_double),
+        ("trail_ms", ctypes.c_double),
+        ("sort_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 _P = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
 
@@ -174,6 +194,12 @@ SIGNATURES = {
     "slm_batch_write_trail": [_P, ctypes.c_uint32, _FP, ctypes.c_uint64],
     "slm_batch_total_mass": [_P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32],
     "slm_batch_get_info": [_P, ctypes.POINTER(BatchInfo)],
+    "slm_get_tuning": [_P, ctypes.POINTER(Tuning)],
+    "slm_set_tuning": [_P, ctypes.POINTER(Tuning)],
+    "slm_autotune": [_P, ctypes.c_uint32, ctypes.POINTER(Tuning)],
+    "slm_batch_get_tuning": [_P, ctypes.POINTER(Tuning)],
+    "slm_batch_set_tuning": [_P, ctypes.POINTER(Tuning)],
+    "slm_batch_autotune": [_P, ctypes.c_uint32, ctypes.POINTER(Tuning)],
 }
 
 _lib = None

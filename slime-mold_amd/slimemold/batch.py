@@ -95,6 +95,25 @@ class Batch:
     def synchronize(self) -> None:
         N.check(N.lib().slm_batch_synchronize(self.handle))
 
+    # -- launch tuning ------------------------------------------------------
+    def get_tuning(self) -> "N.Tuning":
+        t = N.Tuning()
+        N.check(N.lib().slm_batch_get_tuning(self.handle, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, tuning: "N.Tuning" = None, **fields) -> None:
+        """Apply a tuning; keyword fields override those of `tuning` (or of
+        the current one)."""
+        t = N.Tuning.from_buffer_copy(tuning) if tuning is not None else self.get_tuning()
+        for k, v in fields.items():
+            setattr(t, k, v)
+        N.check(N.lib().slm_batch_set_tuning(self.handle, ctypes.byref(t)))
+
+    def autotune(self, trial_steps: int = 20) -> "N.Tuning":
+        t = N.Tuning()
+        N.check(N.lib().slm_batch_autotune(self.handle, trial_steps, ctypes.byref(t)))
+        return t
+
     def set_params(self, k: int, params) -> None:
         p = _params(params)
         N.check(N.lib().slm_batch_set_params(self.handle, k, ctypes.byref(p)))

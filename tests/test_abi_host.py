@@ -128,6 +128,11 @@ def test_null_handle_calls_fail_cleanly(slm):
     assert L.slm_destroy(None) == -1
     assert L.slm_resize(None, 4, 4) == -1
     assert L.slm_total_mass(None, None) == -1
+    t = slm.Tuning()
+    assert L.slm_get_tuning(None, ctypes.byref(t)) == -1
+    assert L.slm_set_tuning(None, ctypes.byref(t)) == -1
+    assert L.slm_autotune(None, 5, ctypes.byref(t)) == -1
+    assert L.slm_batch_autotune(None, 5, None) == -1
     assert "NULL" in L.slm_last_error().decode()
 
 

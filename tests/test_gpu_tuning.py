@@ -1,0 +1,124 @@
+"""Launch tuning and the autotuner on the GPU.
+
+A tuning only changes how the step is laid out (agents per workgroup, trail
+tile rows, re-sort interval and bin size, tiled sensing), never a result bit:
+every tuning, switched mid-run, must track the oracle exactly, and autotune
+must leave the state it found, so a run that autotunes halfway equals one that
+does not.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from parity_util import assert_bitwise, preset
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = [
+    dict(agents_per_group=512),
+    dict(agents_per_group=4096, trail_rows=4),
+    dict(trail_rows=64),
+    dict(resort_interval=3, sort_cell_log2=3),
+    dict(resort_interval=2, sort_cell_log2=5, tiled_sensing=1, agents_per_group=2048),
+    dict(resort_interval=0, tiled_sensing=1, trail_rows=8),
+]
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_every_tuning_switched_mid_run_tracks_the_oracle(slm, orc, storage):
+    s = preset(slm, "Firecracker Trees")  # jitter on: the original index matters
+    w, h, n = 520, 300, 40000
+    with slm.Simulation(w, h, n, seed=21, params=s, storage=storage, graph_chunk=4) as sim:
+        a, t = sim.read_agents(), sim.read_trail()
+        for k, fields in enumerate(TUNINGS):
+            print(f"[tuning] {storage} {fields}", flush=True)
+            sim.set_tuning(**fields)
+            got = sim.get_tuning()
+            for name, v in fields.items():
+                assert getattr(got, name) == v, name
+            sim.step(5 + k)
+            orc.step(a, t, s, 5 + k, storage=storage)
+            assert_bitwise(sim.read_agents(), a, f"agents after {fields}")
+            assert_bitwise(sim.read_trail(), t, f"trail after {fields}")
+        assert sim.info().steps_done == sum(5 + k for k in range(len(TUNINGS)))
+
+
+def test_set_tuning_rejects_bad_values(slm):
+    with slm.Simulation(64, 64, 100, seed=1) as sim:
+        before = sim.get_tuning().as_dict()
+        for fields in [dict(agents_per_group=300), dict(agents_per_group=8192),
+                       dict(agents_per_group=128), dict(trail_rows=3), dict(trail_rows=128),
+                       dict(sort_cell_log2=13), dict(tiled_sensing=2)]:
+            with pytest.raises(slm.SlimeMoldError) as e:
+                sim.set_tuning(**fields)
+            assert e.value.code == -1, fields
+        t = sim.get_tuning()
+        t.reserved[0] = 1
+        assert slm.lib().slm_set_tuning(sim.handle, ctypes.byref(t)) == -1
+        assert sim.get_tuning().as_dict() == before
+        sim.step(3)  # still usable
+
+
+@pytest.mark.parametrize("resort", [0, 4])
+def test_autotune_restores_state(slm, orc, resort):
+    s = preset(slm, "Waves")
+    w, h, n = 640, 480, 60000
+    with slm.Simulation(w, h, n, seed=5, params=s, resort_interval=resort, graph_chunk=10) as sim:
+        sim.step(7)
+        a, t = sim.read_agents(), sim.read_trail()
+        res = sim.autotune(trial_steps=6)
+        print(f"[autotune] {res.as_dict()}", flush=True)
+        assert res.trials >= 10
+        assert res.step_ms > 0 and res.trail_ms > 0 and res.agents_ms > 0
+        assert res.step_ms >= 0.99 * (res.agents_ms + res.trail_ms + res.sort_ms)
+        assert sim.info().steps_done == 7
+        assert_bitwise(sim.read_agents(), a, "agents after autotune")
+        assert_bitwise(sim.read_trail(), t, "trail after autotune")
+        chosen = sim.get_tuning()
+        for k in ("resort_interval", "sort_cell_log2", "agents_per_group", "trail_rows",
+                  "tiled_sensing"):
+            assert getattr(chosen, k) == getattr(res, k), k
+        sim.step(23)
+        orc.step(a, t, s, 23)
+        assert_bitwise(sim.read_agents(), a, "agents 23 steps after autotune")
+        assert_bitwise(sim.read_trail(), t, "trail 23 steps after autotune")
+
+
+def test_batch_autotune_and_tunings_keep_results(slm):
+    specs = [slm.InstanceSpec(w, h, n, seed=40 + k, params=preset(slm, name))
+             for k, (w, h, n, name) in enumerate([(256, 256, 9000, "Default"),
+                                                  (1030, 40, 3000, "Snake"),
+                                                  (512, 128, 0, "Curls"),
+                                                  (64, 96, 5000, "Mesh"),
+                                                  (300, 200, 20000, "Firecracker Trees")])]
+    with slm.Batch(specs, graph_chunk=5) as ref:
+        ref.step(30)
+        want = [(ref.read_agents(k), ref.read_trail(k)) for k in range(len(specs))]
+    with slm.Batch(specs, graph_chunk=5) as b:
+        b.step(10)
+        res = b.autotune(trial_steps=4)
+        print(f"[batch autotune] {res.as_dict()}", flush=True)
+        assert res.trials >= 10
+        b.step(10)
+        b.set_tuning(resort_interval=3, agents_per_group=1024, trail_rows=16)
+        b.step(10)
+        assert b.info().steps_done == 30
+        for k in range(len(specs)):
+            assert_bitwise(b.read_agents(k), want[k][0], f"instance {k} agents")
+            assert_bitwise(b.read_trail(k), want[k][1], f"instance {k} trail")
+
+
+def test_f64_positions_survive_retuning(slm):
+    s = preset(slm, "Default")
+    with slm.Simulation(300, 200, 5000, seed=2, params=s, f64_positions=True) as ref:
+        ref.step(12)
+        wa, wt = ref.read_agents(), ref.read_trail()
+    with slm.Simulation(300, 200, 5000, seed=2, params=s, f64_positions=True) as sim:
+        sim.step(4)
+        sim.set_tuning(resort_interval=2)
+        sim.step(4)
+        sim.set_tuning(resort_interval=0, agents_per_group=512)
+        sim.step(4)
+        assert_bitwise(sim.read_agents(), wa, "agents")
+        assert_bitwise(sim.read_trail(), wt, "trail")

@@ -166,6 +166,33 @@ def test_jitter_range_and_zero(orc):
     assert abs(deltas.mean()) < 0.03 and deltas.std() == pytest.approx(0.5 / math.sqrt(3), rel=0.1)
 
 
+def test_jitter_hash_matches_double_precision_closed_form(orc):
+    """compute.wgsl:115-118: r = fract(sin(f32(i)*12.9898 + x*78.233 + y*37.719) * 43758.5453)
+    with the pre-move position and the agent's index.  The argument is formed in
+    f32 (left to right, each product rounded) and sin is taken in double here;
+    the oracle's f32 sin is within 2 ulp, which the x43758 scale turns into at
+    most ~5e-3, plus the f32 product's own ulp (<= 2^-8 near 4e4): so
+    |r - r_ref| <= 0.02 (circular).  A wrong constant, operand, index or
+    post-move position gives unrelated values and fails."""
+    f = np.float32
+    t = np.zeros((4096, 4096), np.float32)
+    rng = np.random.default_rng(12)
+    J = f(0.5)
+    worst = 0.0
+    for _ in range(400):
+        i = int(rng.integers(0, 1 << 26))
+        x, y = f(rng.uniform(0, 4096)), f(rng.uniform(0, 4096))
+        a, _ = orc.agent_update(t, params(agent_jitter=float(J), agent_speed_min=0.0,
+                                          agent_speed_max=0.0), i, [x, y, 3.0, 0.0])
+        r_got = (float(a[2]) - 3.0) / float(J) * 0.5 + 0.5
+        arg = f(f(f(i) * f(12.9898)) + f(x * f(78.233))) + f(y * f(37.719))
+        p = f(f(math.sin(float(arg))) * f(43758.5453))
+        r_ref = float(p - np.floor(p))
+        d = abs(r_got - r_ref)
+        worst = max(worst, min(d, 1.0 - d))
+    assert worst <= 0.02, worst
+
+
 def test_deposit_cell_is_truncated_position(orc):
     t = np.zeros((16, 16), np.float32)
     a, d = orc.agent_update(t, params(agent_speed_min=0.0, agent_speed_max=0.0), 0,
