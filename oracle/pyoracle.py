@@ -33,6 +33,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         sig = {
             "orc_sincos": (None, [ctypes.c_float, _F, _F]),
+            "orc_set_libm_trig": (None, [ctypes.c_int]),
             "orc_wgsl_mod": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
             "orc_fract": (ctypes.c_float, [ctypes.c_float]),
             "orc_mix": (ctypes.c_float, [ctypes.c_float, ctypes.c_float, ctypes.c_float]),
@@ -98,6 +99,11 @@ def sincos(x: float):
     s, c = ctypes.c_float(), ctypes.c_float()
     lib().orc_sincos(x, ctypes.byref(s), ctypes.byref(c))
     return s.value, c.value
+
+
+def set_libm_trig(on: bool) -> None:
+    """Agent updates use the C library's sinf/cosf (True) or the shared trig."""
+    lib().orc_set_libm_trig(1 if on else 0)
 
 
 def init_agents(seed: int, w: int, h: int, n: int, smin: float, smax: float) -> np.ndarray:

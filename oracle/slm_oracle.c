@@ -24,6 +24,20 @@
 
 void orc_sincos(float x, float* s, float* c) { slm_sincosf(x, s, c); }
 
+/* The agent update's sin/cos: the shared implementation, or (for tests that
+ * model a reference whose transcendental bits differ) the C library's. */
+static int g_libm_trig = 0;
+void orc_set_libm_trig(int on) { g_libm_trig = on != 0; }
+static inline void agent_sincos(float x, float* s, float* c) {
+  if (g_libm_trig) {
+    *s = sinf(x);
+    *c = cosf(x);
+  } else {
+    slm_sincosf(x, s, c);
+  }
+}
+static inline float agent_sin(float x) { return g_libm_trig ? sinf(x) : slm_sinf(x); }
+
 /* WGSL f32 '%': e1 - e2 * trunc(e1 / e2), each operation rounded (S5). */
 float orc_wgsl_mod(float a, float b) {
   float q = a / b;
@@ -185,9 +199,9 @@ int64_t orc_agent_update(const float* t, uint32_t w, uint32_t h, const orc_param
   float sa = p->agent_sensor_angle, sd = p->agent_sensor_distance;
   float al = angle - sa, ar = angle + sa, ac = angle; /* :75-77 */
   float sl, cl, sr, cr, sc, cc;
-  slm_sincosf(al, &sl, &cl);
-  slm_sincosf(ar, &sr, &cr);
-  slm_sincosf(ac, &sc, &cc);
+  agent_sincos(al, &sl, &cl);
+  agent_sincos(ar, &sr, &cr);
+  agent_sincos(ac, &sc, &cc);
   float lx = x + sd * cl, ly = y + sd * sl; /* :79-90 */
   float rx = x + sd * cr, ry = y + sd * sr;
   float cx = x + sd * cc, cy = y + sd * sc;
@@ -210,7 +224,7 @@ int64_t orc_agent_update(const float* t, uint32_t w, uint32_t h, const orc_param
   float hx = (float)(int32_t)index * 12.9898f;
   hx = hx + x * 78.233f;
   hx = hx + y * 37.719f;
-  float rnd = orc_fract(slm_sinf(hx) * 43758.5453f);
+  float rnd = orc_fract(agent_sin(hx) * 43758.5453f);
   angle += (rnd * 2.0f - 1.0f) * p->agent_jitter;
 
   angle = orc_wgsl_mod(angle, ORC_TAU); /* :121-122 */
@@ -218,7 +232,7 @@ int64_t orc_agent_update(const float* t, uint32_t w, uint32_t h, const orc_param
 
   float move = speed * ORC_TIME_STEP; /* :125-127 */
   float ms, mc;
-  slm_sincosf(angle, &ms, &mc);
+  agent_sincos(angle, &ms, &mc);
   x = x + move * mc;
   y = y + move * ms;
 
@@ -272,9 +286,9 @@ int64_t orc_agent_update_p64(const float* t, uint32_t w, uint32_t h, const orc_p
   speed = wgsl_clamp(speed, p->agent_speed_min, p->agent_speed_max);
   float sa = p->agent_sensor_angle, sd = p->agent_sensor_distance;
   float sl, cl, sr, cr, sc, cc;
-  slm_sincosf(angle - sa, &sl, &cl);
-  slm_sincosf(angle + sa, &sr, &cr);
-  slm_sincosf(angle, &sc, &cc);
+  agent_sincos(angle - sa, &sl, &cl);
+  agent_sincos(angle + sa, &sr, &cr);
+  agent_sincos(angle, &sc, &cc);
   float lv = sample_p64(t, w, h, xd + (double)(sd * cl), yd + (double)(sd * sl));
   float rv = sample_p64(t, w, h, xd + (double)(sd * cr), yd + (double)(sd * sr));
   float cv = sample_p64(t, w, h, xd + (double)(sd * cc), yd + (double)(sd * sc));
@@ -291,13 +305,13 @@ int64_t orc_agent_update_p64(const float* t, uint32_t w, uint32_t h, const orc_p
   float hx = (float)(int32_t)index * 12.9898f;
   hx = hx + x * 78.233f;
   hx = hx + y * 37.719f;
-  float rnd = orc_fract(slm_sinf(hx) * 43758.5453f);
+  float rnd = orc_fract(agent_sin(hx) * 43758.5453f);
   angle += (rnd * 2.0f - 1.0f) * p->agent_jitter;
   angle = orc_wgsl_mod(angle, ORC_TAU);
   if (angle < 0.0f) angle = angle + ORC_TAU;
   float move = speed * ORC_TIME_STEP;
   float ms, mc;
-  slm_sincosf(angle, &ms, &mc);
+  agent_sincos(angle, &ms, &mc);
   double fw = (double)w, fh = (double)h;
   xd = xd + (double)(move * mc);
   yd = yd + (double)(move * ms);

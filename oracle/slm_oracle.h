@@ -52,6 +52,10 @@ enum { ORC_STORAGE_F32 = 0, ORC_STORAGE_BF16 = 1, ORC_STORAGE_F16 = 2 };
 
 /* ---- primitives (exposed for the known-answer tests) -------------------- */
 void orc_sincos(float x, float* s, float* c);
+/* Switch the agent update between the shared sin/cos (default, 0) and the C
+ * library's sinf/cosf (1): a stand-in for a reference whose transcendental
+ * bits differ, for statistical (not bitwise) comparisons. */
+void orc_set_libm_trig(int on);
 float orc_wgsl_mod(float a, float b);
 float orc_fract(float v);
 float orc_mix(float a, float b, float t);
