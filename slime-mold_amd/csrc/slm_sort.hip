@@ -77,9 +77,17 @@ __global__ __launch_bounds__(kAgentBlock) void count_kernel(
     s_max = 0u;
   }
   __syncthreads();
-  if (active) {
-    atomicMin(&s_min, k);
-    atomicMax(&s_max, k);
+  // wave min/max by butterfly, then one LDS atomic per wave (256 same-address
+  // LDS atomics per block serialised and dominated the kernel)
+  uint32_t wmin = active ? k : 0xffffffffu, wmax = active ? k : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    wmin = min(wmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(wmin), off)));
+    wmax = max(wmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(wmax), off)));
+  }
+  if ((threadIdx.x & 63u) == 0u) {
+    atomicMin(&s_min, wmin);
+    atomicMax(&s_max, wmax);
   }
   __syncthreads();
   const uint32_t kmin = s_min, kmax = s_max;
