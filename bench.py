@@ -10,16 +10,18 @@ Batched leg: the 256-instance mix (grids 256^2..2048^2, 2^14..2^22 agents,
 parameters swept over the preset range) with a bf16 trail, all instances
 advanced by one agent launch and one trail launch per step.
 
-Before its warmup each leg lets the library's autotuner choose the launch
-tuning (re-sort interval and bin size, agents per workgroup, trail tile rows,
-tiled sensing) on its own state, untimed; the choice is reported under
-"tuning".
+The plain leg runs a fixed launch tuning (flags; by default 16-row trail
+tiles and a re-sort every 20 steps).  Before its warmup the batched (or
+combined) leg lets the library's autotuner choose its tuning (re-sort interval
+and bin size, agents per workgroup, trail tile rows, tiled sensing) on its own
+state, untimed; each leg reports its tuning under "tuning".
 
-Prints one JSON line: value = plain agent-updates/s over the timed region, the
-roofline of the plain leg's dominant kernel (per-kernel device time from HIP
-events on the library's stream, eager launches right after the timed region),
-the diffuse kernel's achieved bandwidth, the batched leg, and the CPU oracle
-timed on a bounded sample of the plain leg's state on the host cores.
+Prints one JSON line: value = plain agent-updates/s over the timed region; the
+roofline of the plain leg's dominant kernel, whose average launch time comes
+from HIP events recorded inside the captured graph chunks of a further
+segment of as many steps, same presets, run right after the timed region; the
+diffuse kernel's achieved bandwidth; the batched leg; and the CPU oracle timed
+on a bounded sample of the plain leg's state on the host cores.
 """
 import argparse
 import json
@@ -40,6 +42,13 @@ STORAGE_BYTES = {"f32": 4, "bf16": 2, "f16": 2}
 # MI355X: FETCH_SIZE reads half the bytes a plain copy moves, WRITE_SIZE all.
 FETCH_SIZE_SCALE = 2.0
 WRITE_SIZE_SCALE = 1.0
+
+
+# HBM bytes per launch of the plain leg's dominant kernel at the default
+# workload (4096^2 f32, 2^24 agents, re-sort every 20 steps, 16-row trail
+# tiles): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, recorded
+# in profiles/agents_trail_c1_counters.md.
+MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (179500 * 2 + 274500) * 1024}
 
 
 def agent_bytes(sb, pos64=False):
@@ -69,12 +78,19 @@ def parse():
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--sort-cell-log2", type=int, default=4, help="re-sort bin edge 2^k cells")
     ap.add_argument("--tiled", action="store_true", help="LDS-staged tiled sensing")
+    ap.add_argument("--trail-rows", type=int, default=16,
+                    help="plain leg: rows per trail-pass tile (0: chosen from the grid)")
+    ap.add_argument("--agents-per-group", type=int, default=256)
     ap.add_argument("--autotune", type=int, default=20,
-                    help="autotune trial steps before each leg's warmup (0: keep the flags' "
-                         "tuning)")
-    ap.add_argument("--timing-steps", type=int, default=30,
-                    help="eager steps per preset for the per-kernel event timing")
+                    help="autotune trial steps before the batched/combined leg's warmup (0: off)")
+    ap.add_argument("--autotune-plain", action="store_true",
+                    help="also autotune the plain leg (otherwise it runs the flags' tuning)")
+    ap.add_argument("--measure-steps", type=int, default=0,
+                    help="steps of the per-kernel timing segment after the timed region "
+                         "(0: as many as the timed region)")
     ap.add_argument("--batch-steps", type=int, default=1000, help="batched leg steps (0: skip)")
+    ap.add_argument("--diffuse-steps", type=int, default=500,
+                    help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-storage", default="bf16", choices=["f32", "bf16", "f16"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -119,10 +135,12 @@ def cpu_baseline(orc, agents, trail, settings, seconds, threads):
     return n, dt
 
 
-def kernel_times(handle, presets, steps, set_params):
-    """Average device time per launch of each kernel over eager steps."""
+def kernel_times(handle, presets, segments, set_params):
+    """Average device time per launch of each kernel, from HIP events recorded
+    inside the same graph chunks slm_step replays (slm_step_timed), over the
+    given presets and step counts."""
     tot = {"agents_ms": 0.0, "trail_ms": 0.0, "sort_ms": 0.0, "agents": 0, "trail": 0, "sort": 0}
-    for p in presets:
+    for p, steps in zip(presets, segments):
         if set_params is not None:
             set_params(p)
         t = handle.step_timed(steps)
@@ -153,10 +171,11 @@ def plain_leg(slm, args):
     sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
                          resort_interval=args.resort, graph_chunk=args.graph_chunk,
                          sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled)
+    sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group)
     log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
         f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}")
     state0 = (sim.read_agents(), sim.read_trail()) if args.cpu_seconds > 0 else None
-    tuning = autotune(sim, args.autotune, "plain")
+    tuning = autotune(sim, args.autotune if args.autotune_plain else 0, "plain")
     sim.step(args.warmup)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -166,12 +185,18 @@ def plain_leg(slm, args):
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"plain: timed {K} steps in {elapsed:.3f} s")
-    a_s, t_s, s_s = kernel_times(sim, presets, args.timing_steps, sim.set_params)
+    M = args.measure_steps or K
+    mseg = [M // 3, M // 3, M - 2 * (M // 3)]
+    t0 = time.perf_counter()
+    a_s, t_s, s_s = kernel_times(sim, presets, mseg, sim.set_params)
+    m_elapsed = time.perf_counter() - t0
+    log(f"plain: measured {M} steps with per-kernel events in {m_elapsed:.3f} s")
     mass = sim.total_mass()
     sim.close()
     a_bytes, t_bytes = agent_bytes(sb) * N, trail_bytes(sb) * W * H
     return {
         "value": N * K / elapsed, "elapsed": elapsed, "steps": K, "segments": seg,
+        "measure_steps": M, "measure_value": N * M / m_elapsed,
         "agents_s": a_s, "trail_s": t_s, "sort_s": s_s, "mass": mass,
         "agents_roof": roofline("agents_kernel", a_bytes, a_s),
         "trail_roof": roofline("trail_kernel", t_bytes, t_s),
@@ -191,6 +216,32 @@ def autotune(handle, trial_steps, label):
     return d
 
 
+def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps):
+    """The trail pass alone on a map larger than the 256 MB Infinity Cache:
+    `side`^2 cells and no agents, so every step is one trail-kernel launch;
+    graph chunks, wall time of `steps` steps between synchronisations (launch
+    gaps included, so a lower bound on the kernel's own bandwidth), after the
+    autotuner has picked the tile rows."""
+    sb = STORAGE_BYTES[storage]
+    sim = slm.Simulation(side, side, 0, seed=3, params=slm.workloads.preset("Default"),
+                         storage=storage, graph_chunk=graph_chunk)
+    tuning = autotune(sim, tune_steps, f"diffuse {side}^2 {storage}")
+    sim.step(2 * graph_chunk)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    dt = time.perf_counter() - t0
+    sim.close()
+    cells = side * side
+    gbps = trail_bytes(sb) * cells * steps / dt / 1e9
+    log(f"diffuse {side}^2 {storage}: {dt / steps * 1e6:.1f} us/step, {gbps:.0f} GB/s")
+    return {"side": side, "storage": storage, "map_bytes": cells * sb, "steps": steps,
+            "us_per_step": round(dt / steps * 1e6, 2), "achieved_gbps": round(gbps, 1),
+            "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
+            "cells_per_s": cells * steps / dt, "trail_rows": tuning["trail_rows"]}
+
+
 def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label,
               tiled=False, tune_steps=0):
     sb = STORAGE_BYTES[storage]
@@ -208,7 +259,8 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
     b.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"{label}: timed {steps} steps in {elapsed:.3f} s")
-    a_s, t_s, s_s = kernel_times(b, [None], 20, None)
+    M = max(1, steps // 5)
+    a_s, t_s, s_s = kernel_times(b, [None], [M], None)
     masses = b.total_mass()
     b.close()
     n_agents, n_cells = info.total_agents, info.total_cells
@@ -224,7 +276,8 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
         "diffuse_gbps": round(trail_bytes(sb) * n_cells / t_s / 1e9, 1),
         "diffuse_pct_hbm_peak": round(100 * trail_bytes(sb) * n_cells / t_s / 1e9 / HBM_PEAK_GBPS, 2),
         "agents_gbps": round(agent_bytes(sb) * n_agents / a_s / 1e9, 1),
-        "graph_chunk": graph_chunk, "tuning": tuning, "total_mass_sum": float(masses.sum()),
+        "graph_chunk": graph_chunk, "tuning": tuning, "measure_steps": M,
+        "total_mass_sum": float(masses.sum()),
     }
 
 
@@ -264,6 +317,10 @@ def main():
     plain = plain_leg(slm, args)
     W, H, N = args.width, args.height, 1 << args.agents_log2
     dominant = plain["agents_roof"] if plain["agents_s"] >= plain["trail_s"] else plain["trail_roof"]
+    dominant = dict(dominant)
+    dominant["traffic"] = MEASURED_TRAFFIC.get(
+        (dominant["kernel"], W, H, N, args.storage)
+        if (args.resort, args.trail_rows, args.agents_per_group) == (20, 16, 256) else None)
     out = {
         "metric": METRIC,
         "value": plain["value"],
@@ -294,7 +351,9 @@ def main():
             "cells_per_s": W * H / plain["trail_s"],
             "avg_launch_us": plain["trail_roof"]["avg_launch_us"],
             "bytes_per_launch": plain["trail_roof"]["bytes_per_launch"],
-            "note": ("trail of this size fits the 256 MB Infinity Cache"
+            "note": ("trail of this size fits the 256 MB Infinity Cache; event records "
+                     "around each kernel add a few microseconds to short kernels; see "
+                     "diffuse_hbm for maps larger than the cache"
                      if plain["trail_roof"]["bytes_per_launch"] <= 256e6 else ""),
         },
         "agents_kernel": {
@@ -304,9 +363,18 @@ def main():
             "bytes_per_launch": plain["agents_roof"]["bytes_per_launch"],
         },
         "sort_avg_launch_us": round(plain["sort_s"] * 1e6, 2),
+        "kernel_timing": (f"HIP events inside the captured graph chunks of a further "
+                          f"{plain['measure_steps']} steps (same presets, thirds) right after the "
+                          f"timed region; those steps ran at "
+                          f"{plain['measure_value']:.4g} agent-updates/s"),
         "final_total_mass": plain["mass"],
         "cpu_baseline": None,
     }
+    if args.diffuse_steps > 0:
+        out["diffuse_hbm"] = [diffuse_leg(slm, 8192, "f32", args.diffuse_steps, args.graph_chunk,
+                                          args.autotune),
+                              diffuse_leg(slm, 16384, "bf16", args.diffuse_steps,
+                                          args.graph_chunk, args.autotune)]
     if args.batch_steps > 0:
         specs = slm.workloads.batch_mix(args.batch_count)
         bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.resort,

@@ -160,8 +160,10 @@ int slm_get_params(const slm_sim* sim, slm_params* out);
  * src/main.rs:1163-1235 (the display pass there only reads the trail). */
 int slm_step(slm_sim* sim, uint32_t n_steps);
 
-/* As slm_step, launched eagerly with HIP events around every kernel; waits
- * and adds the per-kernel device times to *out (zeroed first). */
+/* As slm_step, with HIP events around every kernel; waits and adds the
+ * per-kernel device times to *out (zeroed first).  Whole graph chunks replay
+ * captured graphs holding the event records (the launch path of slm_step);
+ * the remaining steps are launched eagerly. */
 int slm_step_timed(slm_sim* sim, uint32_t n_steps, slm_timing* out);
 
 /* Wait for all work enqueued on the handle's stream. */
@@ -282,10 +284,12 @@ int slm_get_tuning(const slm_sim* sim, slm_tuning* out);
  * are kept; captured graphs are rebuilt on the next slm_step. */
 int slm_set_tuning(slm_sim* sim, const slm_tuning* tuning);
 /* Times candidate tunings on the simulation's own state and keeps the
- * fastest: the trail tile rows by trail-pass time, then the re-sort interval,
- * the bin size, tiled sensing and agents per workgroup by total step time
- * (coordinate descent; every trial runs at least `trial_steps` steps, whole
- * re-sort intervals, from a copy of the current state).  The state the call
+ * fastest: the trail tile rows by trail-pass time, then the re-sort interval
+ * (0, 5, 10, 20 or 40), the bin size, tiled sensing and agents per workgroup
+ * by total step time (coordinate descent).  Every trial starts from a copy of
+ * the current state, warms up 40 steps and is timed over the next
+ * `trial_steps` steps rounded up to a multiple of 40, so all candidates see
+ * the same stretch of the simulation.  The state the call
  * found is restored afterwards, so stepping on gives the same bits as without
  * the call.  out (may be NULL) receives the chosen tuning and its timing.
  * Needs device memory for one extra copy of the agents and the trail, plus the

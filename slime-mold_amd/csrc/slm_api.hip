@@ -215,71 +215,30 @@ class Engine {
     return SLM_OK;
   }
 
-  // Steps with HIP events around every kernel.  Events for up to kTimedBatch
-  // steps are recorded before the host waits, so the stream never drains
-  // between steps and host submission gaps stay out of the kernel times.
+  // Steps with HIP events around every kernel.  Whole graph chunks replay
+  // captured graphs that record the events between their kernels (the same
+  // launch path as slm_step); other steps are launched eagerly, events for up
+  // to kTimedBatch steps recorded before the host waits, so the stream never
+  // drains between steps and host submission gaps stay out of kernel times.
+  static constexpr int kEv = 5;  // sort begin/end, agents begin, agents end = trail begin, trail end
   int step_timed(uint32_t n, slm_timing* out) {
     std::memset(out, 0, sizeof *out);
-    constexpr uint32_t kTimedBatch = 128;
-    constexpr int kEv = 5;  // sort begin/end, agents begin, agents end = trail begin, trail end
-    std::vector<hipEvent_t> ev;
-    int rc = SLM_OK;
-    uint32_t batch = n < kTimedBatch ? n : kTimedBatch;
-    for (uint32_t i = 0; i < batch * kEv && rc == SLM_OK; i++) {
-      hipEvent_t e = nullptr;
-      if (hipEventCreate(&e) != hipSuccess)
-        rc = fail(SLM_ERR_HIP, "hipEventCreate failed");
-      else
-        ev.push_back(e);
+    uint32_t chunk = cfg.graph_chunk;
+    while (n > 0) {
+      if (chunk > 0 && n >= chunk && !timed_graphs_off_) {
+        int rc = run_timed_chunk(out);
+        if (rc == kNoTimedGraph) {
+          timed_graphs_off_ = true;  // this runtime cannot record events in graphs
+          continue;
+        }
+        SLM_TRY(rc);
+        n -= chunk;
+      } else {
+        uint32_t m = chunk > 0 && n >= chunk ? chunk : n;
+        SLM_TRY(run_timed_eager(m, out));
+        n -= m;
+      }
     }
-    std::vector<uint8_t> sorted(batch);
-    Pool p = pool();
-    for (uint32_t done = 0; done < n && rc == SLM_OK;) {
-      uint32_t m = n - done < batch ? n - done : batch;
-      for (uint32_t i = 0; i < m; i++) {
-        hipEvent_t* e = &ev[static_cast<size_t>(i) * kEv];
-        bool sort = sort_due(steps_done) && total_agents_ > 0;
-        sorted[i] = sort;
-        if (sort) {
-          (void)hipEventRecord(e[0], stream);
-          launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
-          (void)hipEventRecord(e[1], stream);
-          acur_ ^= 1;
-          tiles_ready_ = true;
-        }
-        (void)hipEventRecord(e[2], stream);
-        enqueue_agents(p);
-        (void)hipEventRecord(e[3], stream);
-        launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
-        (void)hipEventRecord(e[4], stream);
-        cur_ ^= 1;
-        steps_done++;
-      }
-      hipError_t err = hipEventSynchronize(ev[static_cast<size_t>(m - 1) * kEv + 4]);
-      if (err != hipSuccess) {
-        rc = fail(SLM_ERR_HIP, "step failed: %s", hipGetErrorString(err));
-        break;
-      }
-      for (uint32_t i = 0; i < m; i++) {
-        hipEvent_t* e = &ev[static_cast<size_t>(i) * kEv];
-        float ms = 0.0f;
-        if (sorted[i] && hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) {
-          out->sort_ms += ms;
-          out->sort_launches++;
-        }
-        if (total_agents_ > 0 && hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) {
-          out->agents_ms += ms;
-          out->agents_launches++;
-        }
-        if (hipEventElapsedTime(&ms, e[3], e[4]) == hipSuccess) {
-          out->trail_ms += ms;
-          out->trail_launches++;
-        }
-      }
-      done += m;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    if (rc) return rc;
     SLM_HIP(hipGetLastError());
     return SLM_OK;
   }
@@ -495,6 +454,7 @@ class Engine {
     double step, agents, trail, sort;  // ms per step, sorts amortised
     bool ok;
   };
+  static constexpr uint32_t kTuneSpan = 40;  // warm-up and timing granule of a trial
   uint32_t trials_ = 0;
 
   int take_snapshot(Snapshot& s, bool with_trail) {
@@ -599,16 +559,16 @@ class Engine {
     r = Trial{};
     if (!fits(t)) return SLM_OK;
     SLM_TRY(put_tuning(t, snap));
-    // Warm up for one re-sort interval (at least two steps) so the timed
-    // steps start where a long run would be: the first timed step re-sorts
-    // agents that have drifted for a whole interval.
-    uint32_t k = t.resort_interval;
-    uint32_t warm = k > 2 ? k : 2;
+    // Every candidate warms up for the same kTuneSpan steps and is timed over
+    // the same window (whole kTuneSpan multiples), so all see the same
+    // evolving pattern (agent-pass time drifts by tens of percent over a few
+    // dozen steps as trails form).  Candidate intervals divide kTuneSpan, so
+    // the timed window starts with a re-sort of agents that drifted a whole
+    // interval, as in a long run.
     steps_done = 0;
-    for (uint32_t i = 0; i < warm; i++) enqueue_step(sort_due(steps_done + i));
-    steps_done = k ? k : warm;
-    uint32_t n = steps;
-    if (k) n = (steps + k - 1) / k * k;
+    for (uint32_t i = 0; i < kTuneSpan; i++) enqueue_step(sort_due(i));
+    steps_done = kTuneSpan;
+    uint32_t n = (steps + kTuneSpan - 1) / kTuneSpan * kTuneSpan;
     // the step time is the span from the first launch to the end of the last,
     // gaps included; the kernel times come from the events around each kernel
     hipEvent_t span[2] = {nullptr, nullptr};
@@ -669,7 +629,7 @@ class Engine {
       }
     }
     // 2. re-sort interval, 3. bin size, 4. tiled sensing, 5. agents per group
-    for (uint32_t k : {0u, 5u, 10u, 20u, 40u, 80u}) {
+    for (uint32_t k : {0u, 5u, 10u, 20u, 40u}) {
       if (k == best.resort_interval) continue;
       slm_tuning t = best;
       t.resort_interval = k;
@@ -1032,6 +992,183 @@ class Engine {
     return SLM_OK;
   }
 
+  // Enqueues one step with events e[0..kEv) around its kernels; returns
+  // whether it re-sorted.
+  bool enqueue_step_events(hipEvent_t* e, bool do_sort, const Pool& p) {
+    bool sort = do_sort && sorting() && total_agents_ > 0;
+    if (sort) {
+      (void)hipEventRecord(e[0], stream);
+      launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
+      (void)hipEventRecord(e[1], stream);
+      acur_ ^= 1;
+      tiles_ready_ = true;
+    }
+    (void)hipEventRecord(e[2], stream);
+    enqueue_agents(p);
+    (void)hipEventRecord(e[3], stream);
+    launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
+    (void)hipEventRecord(e[4], stream);
+    cur_ ^= 1;
+    return sort;
+  }
+
+  void add_times(const hipEvent_t* e, bool sorted, slm_timing* out) {
+    float ms = 0.0f;
+    if (sorted && hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) {
+      out->sort_ms += ms;
+      out->sort_launches++;
+    }
+    if (total_agents_ > 0 && hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) {
+      out->agents_ms += ms;
+      out->agents_launches++;
+    }
+    if (hipEventElapsedTime(&ms, e[3], e[4]) == hipSuccess) {
+      out->trail_ms += ms;
+      out->trail_launches++;
+    }
+  }
+
+  static int create_events(std::vector<hipEvent_t>& ev, size_t n) {
+    ev.clear();
+    for (size_t i = 0; i < n; i++) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        destroy_events(ev);
+        return fail(SLM_ERR_HIP, "hipEventCreate failed");
+      }
+      ev.push_back(e);
+    }
+    return SLM_OK;
+  }
+
+  static void destroy_events(std::vector<hipEvent_t>& ev) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+  }
+
+  int run_timed_eager(uint32_t n, slm_timing* out) {
+    constexpr uint32_t kTimedBatch = 128;
+    uint32_t batch = n < kTimedBatch ? n : kTimedBatch;
+    std::vector<hipEvent_t> ev;
+    SLM_TRY(create_events(ev, static_cast<size_t>(batch) * kEv));
+    std::vector<uint8_t> sorted(batch);
+    Pool p = pool();
+    int rc = SLM_OK;
+    for (uint32_t done = 0; done < n && rc == SLM_OK;) {
+      uint32_t m = n - done < batch ? n - done : batch;
+      for (uint32_t i = 0; i < m; i++) {
+        sorted[i] = enqueue_step_events(&ev[static_cast<size_t>(i) * kEv], sort_due(steps_done), p);
+        steps_done++;
+      }
+      hipError_t err = hipEventSynchronize(ev[static_cast<size_t>(m - 1) * kEv + 4]);
+      if (err != hipSuccess) {
+        rc = fail(SLM_ERR_HIP, "step failed: %s", hipGetErrorString(err));
+        break;
+      }
+      for (uint32_t i = 0; i < m; i++) add_times(&ev[static_cast<size_t>(i) * kEv], sorted[i], out);
+      done += m;
+    }
+    destroy_events(ev);
+    return rc;
+  }
+
+  static constexpr int kNoTimedGraph = 1;  // capture with event records unsupported
+
+  struct TimedChunk {
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;
+    std::vector<uint8_t> sorted;
+  };
+  TimedChunk timed_[2][2];
+  bool timed_graphs_off_ = false;
+
+  // A timed chunk is built node by node: HIP drops event records made during
+  // stream capture, so each kernel group (re-sort, agents, trail) is captured
+  // into a child graph and chained with explicit event-record nodes.
+  int build_timed_chunk(TimedChunk& tc, uint32_t chunk) {
+    SLM_TRY(create_events(tc.ev, static_cast<size_t>(chunk) * kEv));
+    tc.sorted.assign(chunk, 0);
+    const int cur0 = cur_, acur0 = acur_;
+    const bool tiles0 = tiles_ready_;
+    const Pool p = pool();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipGraphCreate(&g, 0);
+    hipGraphNode_t prev = nullptr;
+    auto add_event = [&](hipEvent_t ev) {
+      if (e != hipSuccess) return;
+      hipGraphNode_t n = nullptr;
+      e = hipGraphAddEventRecordNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, ev);
+      if (e == hipSuccess) prev = n;
+    };
+    auto add_child = [&](auto&& enqueue) {
+      if (e != hipSuccess) return;
+      hipGraph_t sub = nullptr;
+      e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return;
+      enqueue();
+      e = hipStreamEndCapture(stream, &sub);
+      size_t nodes = 0;
+      if (e == hipSuccess) e = hipGraphGetNodes(sub, nullptr, &nodes);
+      if (e == hipSuccess && nodes > 0) {
+        hipGraphNode_t n = nullptr;
+        e = hipGraphAddChildGraphNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, sub);
+        if (e == hipSuccess) prev = n;
+      }
+      if (sub) (void)hipGraphDestroy(sub);
+    };
+    for (uint32_t j = 0; j < chunk; j++) {
+      hipEvent_t* ev = &tc.ev[static_cast<size_t>(j) * kEv];
+      bool sort = chunk_sorts(j);
+      tc.sorted[j] = sort;
+      if (sort) {
+        add_event(ev[0]);
+        add_child([&] { launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream); });
+        add_event(ev[1]);
+        acur_ ^= 1;
+        tiles_ready_ = true;
+      }
+      add_event(ev[2]);
+      add_child([&] { enqueue_agents(p); });
+      add_event(ev[3]);
+      add_child([&] { launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream); });
+      add_event(ev[4]);
+      cur_ ^= 1;
+    }
+    cur_ = cur0;
+    acur_ = acur0;
+    tiles_ready_ = tiles0;
+    if (e == hipSuccess) e = hipGraphInstantiate(&tc.exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      tc.exec = nullptr;
+      destroy_events(tc.ev);
+      return kNoTimedGraph;
+    }
+    return SLM_OK;
+  }
+
+  int run_timed_chunk(slm_timing* out) {
+    uint32_t chunk = cfg.graph_chunk;
+    TimedChunk& tc = timed_[cur_][acur_];
+    if (!tc.exec) SLM_TRY(build_timed_chunk(tc, chunk));
+    SLM_HIP(hipGraphLaunch(tc.exec, stream));
+    for (uint32_t j = 0; j < chunk; j++) {  // the buffer swaps the captured steps make
+      if (tc.sorted[j]) {
+        acur_ ^= 1;
+        tiles_ready_ = true;
+      }
+      cur_ ^= 1;
+    }
+    steps_done += chunk;
+    hipError_t err = hipEventSynchronize(tc.ev[static_cast<size_t>(chunk - 1) * kEv + 4]);
+    if (err != hipSuccess) return fail(SLM_ERR_HIP, "step failed: %s", hipGetErrorString(err));
+    for (uint32_t j = 0; j < chunk; j++)
+      add_times(&tc.ev[static_cast<size_t>(j) * kEv], tc.sorted[j] != 0, out);
+    return SLM_OK;
+  }
+
   void drop_graphs() {
     for (auto& row : graphs_)
       for (auto& g : row)
@@ -1039,6 +1176,13 @@ class Engine {
           (void)hipGraphExecDestroy(g);
           g = nullptr;
         }
+    for (auto& row : timed_)
+      for (auto& tc : row) {
+        if (tc.exec) (void)hipGraphExecDestroy(tc.exec);
+        tc.exec = nullptr;
+        destroy_events(tc.ev);
+        tc.sorted.clear();
+      }
   }
 };
 

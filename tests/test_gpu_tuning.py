@@ -122,3 +122,27 @@ def test_f64_positions_survive_retuning(slm):
         sim.step(4)
         assert_bitwise(sim.read_agents(), wa, "agents")
         assert_bitwise(sim.read_trail(), wt, "trail")
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_step_timed_matches_step(slm, storage):
+    """slm_step_timed replays graph chunks with event-record nodes (and eager
+    steps for the remainder): same bits as slm_step, every kernel timed."""
+    s = preset(slm, "Firecracker Trees")
+    kw = dict(seed=8, params=s, storage=storage, graph_chunk=10, resort_interval=3)
+    with slm.Simulation(300, 200, 20000, **kw) as a, slm.Simulation(300, 200, 20000, **kw) as b:
+        a.step(35)
+        t = b.step_timed(35)
+        t2 = b.step_timed(20)  # replays the cached timed graphs
+        a.step(20)
+        assert_bitwise(b.read_agents(), a.read_agents(), "agents")
+        assert_bitwise(b.read_trail(), a.read_trail(), "trail")
+        for tm in (t, t2):
+            print(f"[timed] {tm.agents_ms:.3f} {tm.trail_ms:.3f} {tm.sort_ms:.3f} "
+                  f"{tm.agents_launches} {tm.trail_launches} {tm.sort_launches}", flush=True)
+        assert t.agents_launches == 35 and t.trail_launches == 35 and t2.trail_launches == 20
+        assert t.sort_launches >= 11 and t2.sort_launches >= 6
+        assert t.agents_ms > 0 and t.trail_ms > 0 and t.sort_ms > 0
+        assert t2.agents_ms > 0 and t2.trail_ms > 0
+        # per launch, times are those of real kernels (not stale or zero events)
+        assert 0.0005 < t.trail_ms / t.trail_launches < 5.0
