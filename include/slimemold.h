@@ -301,6 +301,51 @@ int slm_batch_get_tuning(const slm_batch* batch, slm_tuning* out);
 int slm_batch_set_tuning(slm_batch* batch, const slm_tuning* tuning);
 int slm_batch_autotune(slm_batch* batch, uint32_t trial_steps, slm_tuning* out);
 
+/* ---- display colour map ------------------------------------------------------
+ * Replaces the display compute pass (src/display.wgsl:29-87, dispatched at
+ * src/main.rs:1199-1215) and the LUT buffer it reads (src/main.rs:330-342).
+ * lut: 768 bytes, R[256] then G[256] then B[256], the layout of the reference's
+ * .lut files and LutData (src/lut_manager.rs:4-18,162-186).  Renders the
+ * current trail into a tex_w x tex_h RGBA8 image, row-major, 4 bytes per pixel
+ * (r, g, b, 255): the grid is fitted into the texture keeping its aspect ratio
+ * and centred (f32 scale and offsets as the shader computes them), each pixel
+ * takes its cell by truncation, the intensity clamp(T, 0, 1) selects table
+ * entry u32(intensity * 255); pixels outside the grid are (0, 0, 0, 255).
+ * 1 <= tex_w, tex_h <= 2^24 and tex_w * tex_h <= 2^30. */
+int slm_render_rgba8(slm_sim* sim, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
+                     uint8_t* rgba);
+/* The same into device memory (tex_w * tex_h * 4 bytes on the handle's
+ * device), enqueued on the handle's stream: slm_synchronize waits for it. */
+int slm_render_rgba8_device(slm_sim* sim, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
+                            void* rgba_device);
+int slm_batch_render_rgba8(slm_batch* batch, uint32_t index, const uint8_t* lut, uint32_t tex_w,
+                           uint32_t tex_h, uint8_t* rgba);
+
+/* ---- snapshots ------------------------------------------------------------------
+ * A resumable copy of a single simulation.  Loading it into a simulation of
+ * the same shape (grid, agent count, storage, position precision), or creating
+ * one from it, and stepping on gives bit for bit the result of the run that
+ * was never interrupted.  No counterpart in the reference, which keeps no
+ * state across runs.
+ * Layout, little-endian: a 128-byte header
+ *   char magic[8] = "SLMSNAP1"; uint32 version = 1; uint32 header_bytes = 128;
+ *   uint32 width, height; uint64 agent_count; uint64 seed; int32 storage;
+ *   uint32 flags (SLM_FLAG_F64_POSITIONS or 0); uint64 steps_done;
+ *   uint64 speed_reassignments; slm_params params (44 bytes); uint32 0;
+ *   uint64 agent_bytes; uint64 trail_bytes;
+ * then agent_count f32 records (x, y, angle, speed) in original order; then,
+ * with f64 positions, agent_count f64 (x, y) pairs; then width*height trail
+ * elements in the storage format, row-major; then a uint64 checksum, FNV-1a
+ * over all preceding bytes taken as little-endian 64-bit words (the last one
+ * zero-padded). */
+int slm_snapshot_size(const slm_sim* sim, uint64_t* bytes);
+int slm_save_snapshot(slm_sim* sim, void* buf, uint64_t bytes);
+int slm_load_snapshot(slm_sim* sim, const void* buf, uint64_t bytes);
+/* A new simulation (default launch tuning, 100-step graphs) holding the
+ * snapshot's state. */
+int slm_create_from_snapshot(const void* buf, uint64_t bytes, int32_t device, slm_sim** out);
+
+
 #ifdef __cplusplus
 }
 #endif

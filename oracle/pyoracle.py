@@ -69,6 +69,8 @@ def lib():
                                           ctypes.POINTER(OrcParams), _D, ctypes.c_int]),
             "orc_total_mass": (ctypes.c_double, [_F, ctypes.c_uint64]),
             "orc_total_mass_f64": (ctypes.c_double, [_D, ctypes.c_uint64]),
+            "orc_render": (None, [_F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -217,4 +219,15 @@ def round_storage(values: np.ndarray, storage: str) -> np.ndarray:
     flat_in, flat_out = v.reshape(-1), out.reshape(-1)
     for i in range(flat_in.size):
         flat_out[i] = f(float(flat_in[i]), STORAGE[storage])
+    return out
+
+
+def render(trail: np.ndarray, lut, tex_w: int, tex_h: int) -> np.ndarray:
+    """RGBA8 image (tex_h, tex_w, 4) of the trail through a 768-byte LUT."""
+    t = np.ascontiguousarray(trail, dtype=np.float32)
+    h, w = t.shape
+    lut = np.ascontiguousarray(np.frombuffer(bytes(lut), dtype=np.uint8))
+    assert lut.size == 768
+    out = np.empty((tex_h, tex_w, 4), np.uint8)
+    lib().orc_render(t.ctypes.data_as(_F), w, h, lut.ctypes.data, tex_w, tex_h, out.ctypes.data)
     return out

@@ -561,3 +561,40 @@ double orc_total_mass_f64(const double* t, uint64_t cells) {
   for (uint64_t c = 0; c < cells; c++) s += t[c];
   return s;
 }
+
+/* src/display.wgsl:29-87: fit the grid into a tex_w x tex_h texture keeping
+ * its aspect (f32 scale and offsets), nearest cell by truncation, LUT entry
+ * u32(clamp(T, 0, 1) * 255) of the 768-byte R[256] G[256] B[256] table, alpha
+ * 255; outside the grid black.  rgba: tex_w * tex_h * 4 bytes, row-major. */
+void orc_render(const float* trail, uint32_t w, uint32_t h, const uint8_t* lut, uint32_t tex_w,
+                uint32_t tex_h, uint8_t* rgba) {
+  float sim_w = (float)w, sim_h = (float)h, tw = (float)tex_w, th = (float)tex_h;
+  float sim_aspect = sim_w / sim_h;                   /* :55 */
+  float tex_aspect = tw / th;                         /* :56 */
+  float scale, off_x = 0.0f, off_y = 0.0f;
+  if (tex_aspect > sim_aspect) {                      /* :62-66 */
+    scale = th / sim_h;
+    off_x = (tw - sim_w * scale) * 0.5f;
+  } else {                                            /* :67-71 */
+    scale = tw / sim_w;
+    off_y = (th - sim_h * scale) * 0.5f;
+  }
+  for (uint32_t py = 0; py < tex_h; py++) {
+    for (uint32_t px = 0; px < tex_w; px++) {
+      float fx = ((float)px - off_x) / scale;         /* :74-75 */
+      float fy = ((float)py - off_y) / scale;
+      uint8_t* o = rgba + 4 * ((size_t)py * tex_w + px);
+      o[0] = o[1] = o[2] = 0;
+      o[3] = 255;
+      if (fx >= 0.0f && fx < sim_w && fy >= 0.0f && fy < sim_h) { /* :78 */
+        int32_t x = orc_f32_to_i32(fx), y = orc_f32_to_i32(fy);
+        float t = trail[(size_t)y * w + (size_t)x];
+        t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);  /* :83, :31 */
+        uint32_t k = (uint32_t)(t * 255.0f);            /* :34 */
+        o[0] = lut[k];                                  /* :37-39 */
+        o[1] = lut[k + 256];
+        o[2] = lut[k + 512];
+      }
+    }
+  }
+}

@@ -113,6 +113,10 @@ int orc_step_f64(uint32_t w, uint32_t h, uint64_t n, double* agents_aos, double*
 double orc_total_mass(const float* trail, uint64_t cells);
 double orc_total_mass_f64(const double* trail, uint64_t cells);
 
+/* Display colour map (src/display.wgsl:29-87) into tex_w * tex_h RGBA8. */
+void orc_render(const float* trail, uint32_t w, uint32_t h, const uint8_t* lut, uint32_t tex_w,
+                uint32_t tex_h, uint8_t* rgba);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,7 @@
 // throws).  Field names and preset values are the reference's.
 #pragma once
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -152,6 +153,34 @@ inline PresetManager init_preset_manager() {
 }
 }  // namespace presets
 
+namespace lut {
+// src/lut_manager.rs:4-18: 768 bytes, red[256] then green[256] then blue[256].
+struct LutData {
+  std::string name;
+  std::vector<uint8_t> red, green, blue;
+
+  static LutData from_bytes(const std::string& name, const std::vector<uint8_t>& data) {
+    if (data.size() != 768) throw Error(SLM_ERR_INVALID_ARGUMENT, "Invalid LUT file size");
+    return {name, std::vector<uint8_t>(data.begin(), data.begin() + 256),
+            std::vector<uint8_t>(data.begin() + 256, data.begin() + 512),
+            std::vector<uint8_t>(data.begin() + 512, data.end())};
+  }
+  void reverse() {
+    std::reverse(red.begin(), red.end());
+    std::reverse(green.begin(), green.end());
+    std::reverse(blue.begin(), blue.end());
+  }
+  // The layout the renderer takes (src/main.rs:330-334).
+  std::vector<uint8_t> bytes() const {
+    std::vector<uint8_t> out(red);
+    out.insert(out.end(), green.begin(), green.end());
+    out.insert(out.end(), blue.begin(), blue.end());
+    return out;
+  }
+};
+}  // namespace lut
+
+using lut::LutData;
 using presets::init_preset_manager;
 using presets::Preset;
 using presets::PresetManager;
@@ -231,9 +260,31 @@ class Simulation {
     check(slm_autotune(h_, trial_steps, &t));
     return t;
   }
+  // Display colour map: tex_h x tex_w RGBA8, row-major (src/display.wgsl:29-87).
+  std::vector<uint8_t> render(const LutData& lut, uint32_t tex_w, uint32_t tex_h) {
+    std::vector<uint8_t> table = lut.bytes(), out(static_cast<size_t>(tex_w) * tex_h * 4);
+    check(slm_render_rgba8(h_, table.data(), tex_w, tex_h, out.data()));
+    return out;
+  }
+  std::vector<uint8_t> snapshot() {
+    uint64_t n = 0;
+    check(slm_snapshot_size(h_, &n));
+    std::vector<uint8_t> buf(n);
+    check(slm_save_snapshot(h_, buf.data(), n));
+    return buf;
+  }
+  void load_snapshot(const std::vector<uint8_t>& buf) {
+    check(slm_load_snapshot(h_, buf.data(), buf.size()));
+  }
+  static Simulation from_snapshot(const std::vector<uint8_t>& buf, int device = 0) {
+    slm_sim* h = nullptr;
+    check(slm_create_from_snapshot(buf.data(), buf.size(), device, &h));
+    return Simulation(h);
+  }
   slm_sim* handle() { return h_; }
 
  private:
+  explicit Simulation(slm_sim* h) : h_(h) {}
   slm_sim* h_ = nullptr;
 };
 

@@ -624,6 +624,25 @@ __global__ void unpack_agents_kernel(AgentSoA a, uint32_t off, uint32_t n, int p
   if (a.id) a.id[j] = i;
 }
 
+// Snapshot helpers: (x64, y64) pairs in original order.
+__global__ void pack_positions64_kernel(AgentSoA a, uint32_t off, uint32_t n, double2* xy) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint32_t j = off + s;
+  uint32_t i = a.id ? a.id[j] : s;
+  xy[i] = make_double2(a.x64[j], a.y64[j]);
+}
+
+__global__ void unpack_positions64_kernel(AgentSoA a, uint32_t off, uint32_t n,
+                                          const double2* xy) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t j = off + i;  // after unpack_agents_kernel: slot j holds agent i
+  double2 v = xy[i];
+  a.x64[j] = v.x;
+  a.y64[j] = v.y;
+}
+
 template <int S>
 void launch_agents_t(const Pool& pool, const AgentArgs& args, hipStream_t stream) {
   uint32_t nb = blocks_for(pool.total_agents, args.group);
@@ -716,6 +735,20 @@ void launch_unpack_agents(const Instance& in, const AgentSoA& a, int pos64, cons
   if (in.agent_count == 0) return;
   unpack_agents_kernel<<<blocks_for(in.agent_count, 256), 256, 0, stream>>>(
       a, in.agent_off, in.agent_count, pos64, reinterpret_cast<const float4*>(aos));
+}
+
+void launch_pack_positions64(const Instance& in, const AgentSoA& a, double* xy,
+                             hipStream_t stream) {
+  if (in.agent_count == 0) return;
+  pack_positions64_kernel<<<blocks_for(in.agent_count, 256), 256, 0, stream>>>(
+      a, in.agent_off, in.agent_count, reinterpret_cast<double2*>(xy));
+}
+
+void launch_unpack_positions64(const Instance& in, const AgentSoA& a, const double* xy,
+                               hipStream_t stream) {
+  if (in.agent_count == 0) return;
+  unpack_positions64_kernel<<<blocks_for(in.agent_count, 256), 256, 0, stream>>>(
+      a, in.agent_off, in.agent_count, reinterpret_cast<const double2*>(xy));
 }
 
 }  // namespace slm

@@ -137,6 +137,93 @@ int check_engine_config(const EngineConfig& c, const uint32_t* reserved, size_t 
   return SLM_OK;
 }
 
+// ---- snapshot format ----------------------------------------------------------
+constexpr char kSnapMagic[8] = {'S', 'L', 'M', 'S', 'N', 'A', 'P', '1'};
+constexpr uint32_t kSnapVersion = 1;
+
+struct SnapHeader {  // 128 bytes, little-endian
+  char magic[8];
+  uint32_t version;
+  uint32_t header_bytes;
+  uint32_t width, height;
+  uint64_t agent_count;
+  uint64_t seed;
+  int32_t storage;
+  uint32_t flags;  // SLM_FLAG_F64_POSITIONS
+  uint64_t steps_done;
+  uint64_t reassigns;
+  slm_params params;
+  uint32_t reserved;
+  uint64_t agent_bytes;
+  uint64_t trail_bytes;
+};
+static_assert(sizeof(SnapHeader) == 128, "snapshot header layout");
+
+// FNV-1a over little-endian 64-bit words (the tail zero-padded).
+uint64_t snapshot_checksum(const uint8_t* p, uint64_t n) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  uint64_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, p + i, 8);
+    h = (h ^ w) * 0x100000001b3ull;
+  }
+  if (i < n) {
+    uint64_t w = 0;
+    std::memcpy(&w, p + i, n - i);
+    h = (h ^ w) * 0x100000001b3ull;
+  }
+  return h;
+}
+
+float decode_storage(uint32_t bits, int storage) {
+  if (storage == SLM_STORAGE_F32) {
+    float v;
+    std::memcpy(&v, &bits, 4);
+    return v;
+  }
+  if (storage == SLM_STORAGE_BF16) {
+    uint32_t w = bits << 16;
+    float v;
+    std::memcpy(&v, &w, 4);
+    return v;
+  }
+  _Float16 h;
+  uint16_t b16 = static_cast<uint16_t>(bits);
+  std::memcpy(&h, &b16, 2);
+  return static_cast<float>(h);
+}
+
+// Structure and checksum of a snapshot buffer (not its shape).
+int check_snapshot(const uint8_t* buf, uint64_t bytes, SnapHeader* out) {
+  if (!buf) return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot buffer is NULL");
+  if (bytes < 128 + 8) return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot too short");
+  SnapHeader hd;
+  std::memcpy(&hd, buf, sizeof hd);
+  if (std::memcmp(hd.magic, kSnapMagic, 8) != 0)
+    return fail(SLM_ERR_INVALID_ARGUMENT, "not a slime-mold snapshot");
+  if (hd.version != kSnapVersion || hd.header_bytes != 128)
+    return fail(SLM_ERR_UNSUPPORTED, "snapshot version %u not supported", hd.version);
+  if (hd.storage < SLM_STORAGE_F32 || hd.storage > SLM_STORAGE_F16 ||
+      (hd.flags & ~SLM_FLAG_F64_POSITIONS) || hd.reserved)
+    return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot header is malformed");
+  uint64_t n = hd.agent_count;
+  uint64_t cells = static_cast<uint64_t>(hd.width) * hd.height;
+  uint64_t want_agents = n * 16u + ((hd.flags & SLM_FLAG_F64_POSITIONS) ? n * 16u : 0u);
+  uint64_t sb = hd.storage == SLM_STORAGE_F32 ? 4 : 2;
+  if (n > kMaxAgents || hd.width == 0 || hd.height == 0 || hd.width > kMaxSide ||
+      hd.height > kMaxSide || cells > kMaxCells || hd.agent_bytes != want_agents ||
+      hd.trail_bytes != cells * sb || bytes != 128 + want_agents + cells * sb + 8)
+    return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot sizes are inconsistent");
+  uint64_t sum;
+  std::memcpy(&sum, buf + bytes - 8, 8);
+  if (sum != snapshot_checksum(buf, bytes - 8))
+    return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot checksum mismatch");
+  if (!finite_params(hd.params)) return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot params not finite");
+  if (out) *out = hd;
+  return SLM_OK;
+}
+
 struct InstanceSpec {
   uint32_t width, height;
   uint64_t agents;
@@ -383,6 +470,142 @@ class Engine {
     SLM_TRY(layout_trail_tables());
     init_agents(0);
     return sync_checked("agent re-initialisation");
+  }
+
+  // ---- display colour map -------------------------------------------------------
+  int render(size_t k, const uint8_t* lut768, uint32_t tex_w, uint32_t tex_h, void* out,
+             bool device_out) {
+    LutBytes lut;
+    std::memcpy(lut.word, lut768, sizeof lut.word);
+    RenderArgs a{};
+    render_fit(spec[k].width, spec[k].height, tex_w, tex_h, a);
+    a.trail = trail_ptr(cur_, k);
+    if (device_out) {
+      a.out = static_cast<uint32_t*>(out);
+      launch_render(a, lut, cfg.storage, stream);
+      SLM_HIP(hipGetLastError());
+      return SLM_OK;
+    }
+    uint64_t bytes = static_cast<uint64_t>(tex_w) * tex_h * 4u;
+    void* staging = nullptr;
+    SLM_TRY(scratch_alloc(&staging, bytes));
+    a.out = static_cast<uint32_t*>(staging);
+    launch_render(a, lut, cfg.storage, stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(out, staging, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(staging);
+    if (e != hipSuccess) return fail(SLM_ERR_HIP, "render failed: %s", hipGetErrorString(e));
+    return SLM_OK;
+  }
+
+  // ---- snapshots (single simulation) -------------------------------------------
+  // Layout, little-endian: a 128-byte header (see slm_snapshot_size in
+  // include/slimemold.h), the agents as n f32 records (x, y, angle, speed) in
+  // original order, with f64 positions also n (x, y) f64 pairs, the trail as
+  // width*height raw storage elements (row-major), and a 64-bit checksum.
+  static constexpr uint64_t kSnapHeader = 128;
+  uint64_t snap_agent_bytes() const {
+    uint64_t n = spec[0].agents;
+    return n * 16u + (pos64() ? n * 16u : 0u);
+  }
+  uint64_t snap_trail_bytes() const { return cells(0) * storage_bytes(cfg.storage); }
+  uint64_t snapshot_bytes() const {
+    return kSnapHeader + snap_agent_bytes() + snap_trail_bytes() + 8u;
+  }
+
+  int save_snapshot(uint8_t* buf) {
+    const uint64_t n = spec[0].agents;
+    SnapHeader hd;
+    std::memset(&hd, 0, sizeof hd);
+    std::memcpy(hd.magic, kSnapMagic, 8);
+    hd.version = kSnapVersion;
+    hd.header_bytes = kSnapHeader;
+    hd.width = spec[0].width;
+    hd.height = spec[0].height;
+    hd.agent_count = n;
+    hd.seed = spec[0].seed;
+    hd.storage = cfg.storage;
+    hd.flags = pos64() ? SLM_FLAG_F64_POSITIONS : 0u;
+    hd.steps_done = steps_done;
+    hd.reassigns = reassigns[0];
+    hd.params = spec[0].params;
+    hd.agent_bytes = snap_agent_bytes();
+    hd.trail_bytes = snap_trail_bytes();
+    std::memcpy(buf, &hd, sizeof hd);
+    uint8_t* at = buf + kSnapHeader;
+    std::vector<float> aos(n * 4);
+    SLM_TRY(read_agents(0, aos.data()));
+    std::memcpy(at, aos.data(), n * 16u);
+    at += n * 16u;
+    if (pos64() && n > 0) {
+      void* staging = nullptr;
+      SLM_TRY(scratch_alloc(&staging, n * 16u));
+      launch_pack_positions64(inst[0], agents_[acur_], static_cast<double*>(staging), stream);
+      hipError_t e = hipStreamSynchronize(stream);
+      if (e == hipSuccess) e = hipMemcpy(at, staging, n * 16u, hipMemcpyDeviceToHost);
+      (void)hipFree(staging);
+      if (e != hipSuccess) return fail(SLM_ERR_HIP, "snapshot failed: %s", hipGetErrorString(e));
+    }
+    at += pos64() ? n * 16u : 0u;
+    SLM_TRY(sync());
+    SLM_HIP(hipMemcpy(at, trail_ptr(cur_, 0), snap_trail_bytes(), hipMemcpyDeviceToHost));
+    at += snap_trail_bytes();
+    uint64_t sum = snapshot_checksum(buf, snapshot_bytes() - 8u);
+    std::memcpy(at, &sum, 8);
+    return SLM_OK;
+  }
+
+  // `buf` has passed check_snapshot and matches this simulation's shape.
+  int load_snapshot(const uint8_t* buf) {
+    SnapHeader hd;
+    std::memcpy(&hd, buf, sizeof hd);
+    const uint64_t n = hd.agent_count;
+    const uint8_t* at = buf + kSnapHeader;
+    std::vector<float> aos(n * 4);
+    std::memcpy(aos.data(), at, n * 16u);
+    at += n * 16u;
+    std::vector<double> xy;
+    if (pos64()) {
+      xy.resize(n * 2);
+      std::memcpy(xy.data(), at, n * 16u);
+      at += n * 16u;
+      double fw = spec[0].width, fh = spec[0].height;
+      for (uint64_t i = 0; i < n; i++)
+        if (!(xy[2 * i] >= 0.0 && xy[2 * i] <= fw && xy[2 * i + 1] >= 0.0 && xy[2 * i + 1] <= fh))
+          return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot agent %llu: f64 position out of range",
+                      static_cast<unsigned long long>(i));
+    }
+    const uint64_t c = cells(0);
+    const int sb = static_cast<int>(storage_bytes(cfg.storage));
+    for (uint64_t i = 0; i < c; i++) {
+      uint32_t bits = 0;
+      std::memcpy(&bits, at + i * sb, sb);
+      float v = decode_storage(bits, cfg.storage);
+      if ((bits >> (8 * sb - 1)) != 0u || !(v >= 0.0f) || !(v <= kMaxTrailValue))
+        return fail(SLM_ERR_INVALID_ARGUMENT, "snapshot trail cell %llu must lie in [0, 2^100]",
+                    static_cast<unsigned long long>(i));
+    }
+    SLM_TRY(write_agents(0, aos.data()));  // validates positions, restores original order
+    if (pos64() && n > 0) {
+      void* staging = nullptr;
+      SLM_TRY(scratch_alloc(&staging, n * 16u));
+      hipError_t e = hipMemcpy(staging, xy.data(), n * 16u, hipMemcpyHostToDevice);
+      if (e == hipSuccess) {
+        launch_unpack_positions64(inst[0], agents_[acur_], static_cast<const double*>(staging),
+                                  stream);
+        e = hipStreamSynchronize(stream);
+      }
+      (void)hipFree(staging);
+      if (e != hipSuccess) return fail(SLM_ERR_HIP, "snapshot load failed: %s", hipGetErrorString(e));
+    }
+    SLM_HIP(hipMemcpy(trail_ptr(cur_, 0), at, snap_trail_bytes(), hipMemcpyHostToDevice));
+    spec[0].seed = hd.seed;
+    reassigns[0] = hd.reassigns;
+    steps_done = hd.steps_done;
+    tiles_ready_ = false;
+    SLM_TRY(set_params(0, hd.params));
+    return sync_checked("snapshot load");
   }
 
   // ---- launch tuning ----------------------------------------------------------
@@ -1447,6 +1670,80 @@ int slm_autotune(slm_sim* s, uint32_t trial_steps, slm_tuning* out) {
   return rc;
 }
 
+static int check_render(const uint8_t* lut, uint32_t tex_w, uint32_t tex_h, const void* out) {
+  if (!lut || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (tex_w == 0 || tex_h == 0 || tex_w > kMaxSide || tex_h > kMaxSide ||
+      static_cast<uint64_t>(tex_w) * tex_h > (1ull << 30))
+    return fail(SLM_ERR_INVALID_ARGUMENT, "texture must be 1..2^24 on a side, <= 2^30 pixels");
+  return SLM_OK;
+}
+
+int slm_render_rgba8(slm_sim* s, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
+                     uint8_t* rgba) {
+  SIM_GUARD(s);
+  SLM_TRY(check_render(lut, tex_w, tex_h, rgba));
+  return s->eng->render(0, lut, tex_w, tex_h, rgba, false);
+}
+
+int slm_render_rgba8_device(slm_sim* s, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
+                            void* rgba_device) {
+  SIM_GUARD(s);
+  SLM_TRY(check_render(lut, tex_w, tex_h, rgba_device));
+  return s->eng->render(0, lut, tex_w, tex_h, rgba_device, true);
+}
+
+int slm_snapshot_size(const slm_sim* s, uint64_t* bytes) {
+  if (!s || !s->eng || !bytes) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  *bytes = s->eng->snapshot_bytes();
+  return SLM_OK;
+}
+
+int slm_save_snapshot(slm_sim* s, void* buf, uint64_t bytes) {
+  SIM_GUARD(s);
+  if (!buf) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
+  SLM_TRY(check_n(bytes, s->eng->snapshot_bytes(), "save_snapshot (bytes != slm_snapshot_size)"));
+  return s->eng->save_snapshot(static_cast<uint8_t*>(buf));
+}
+
+int slm_load_snapshot(slm_sim* s, const void* buf, uint64_t bytes) {
+  SIM_GUARD(s);
+  SnapHeader hd;
+  SLM_TRY(check_snapshot(static_cast<const uint8_t*>(buf), bytes, &hd));
+  const Engine& e = *s->eng;
+  if (hd.width != e.spec[0].width || hd.height != e.spec[0].height ||
+      hd.agent_count != e.spec[0].agents || hd.storage != e.cfg.storage ||
+      ((hd.flags & SLM_FLAG_F64_POSITIONS) != 0) != e.pos64())
+    return fail(SLM_ERR_INVALID_ARGUMENT,
+                "snapshot shape (grid, agents, storage, positions) differs from the simulation's");
+  return s->eng->load_snapshot(static_cast<const uint8_t*>(buf));
+}
+
+int slm_create_from_snapshot(const void* buf, uint64_t bytes, int32_t device, slm_sim** out) {
+  if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  SnapHeader hd;
+  SLM_TRY(check_snapshot(static_cast<const uint8_t*>(buf), bytes, &hd));
+  slm_config cfg;
+  slm_config_default(&cfg);
+  cfg.width = hd.width;
+  cfg.height = hd.height;
+  cfg.agent_count = hd.agent_count;
+  cfg.seed = hd.seed;
+  cfg.storage = hd.storage;
+  cfg.device = device;
+  cfg.flags = hd.flags;
+  slm_sim* s = nullptr;
+  SLM_TRY(slm_create(&cfg, &hd.params, &s));
+  int rc = slm_load_snapshot(s, buf, bytes);
+  if (rc) {
+    std::string msg = g_last_error;
+    slm_destroy(s);
+    return fail(rc, "%s", msg.c_str());
+  }
+  *out = s;
+  return SLM_OK;
+}
+
 // ---- batches ------------------------------------------------------------------
 
 int slm_batch_config_default(slm_batch_config* cfg) {
@@ -1608,6 +1905,14 @@ int slm_batch_autotune(slm_batch* b, uint32_t trial_steps, slm_tuning* out) {
   int rc = b->eng->autotune(trial_steps, out);
   sync_batch_config(b->cfg, *b->eng);
   return rc;
+}
+
+int slm_batch_render_rgba8(slm_batch* b, uint32_t k, const uint8_t* lut, uint32_t tex_w,
+                           uint32_t tex_h, uint8_t* rgba) {
+  BATCH_GUARD(b);
+  BATCH_INDEX(b, k);
+  SLM_TRY(check_render(lut, tex_w, tex_h, rgba));
+  return b->eng->render(k, lut, tex_w, tex_h, rgba, false);
 }
 
 int slm_batch_get_info(const slm_batch* b, slm_batch_info* out) {

@@ -141,6 +141,28 @@ constexpr int kMassBlocks = 256;
 void launch_total_mass(const Pool& pool, const void* trail, double* partials, double* out,
                        hipStream_t stream);
 
+// ---- display colour map --------------------------------------------------
+struct LutBytes {
+  uint32_t word[192];  // the 768 table bytes R[256] G[256] B[256], little-endian words
+};
+struct RenderArgs {
+  const void* trail;  // one instance's trail
+  uint32_t width;     // cells per trail row
+  float sim_w, sim_h;
+  uint32_t tex_w, tex_h;
+  float scale, off_x, off_y;
+  uint32_t* out;      // tex_w * tex_h RGBA8 pixels, row-major
+};
+// Fills the fit fields of `a` as src/display.wgsl:48-72 computes them.
+void render_fit(uint32_t width, uint32_t height, uint32_t tex_w, uint32_t tex_h, RenderArgs& a);
+void launch_render(const RenderArgs& a, const LutBytes& lut, int storage, hipStream_t stream);
+
+// ---- snapshots: f64 positions in original order ----------------------------
+void launch_pack_positions64(const Instance& in, const AgentSoA& a, double* xy,
+                             hipStream_t stream);
+void launch_unpack_positions64(const Instance& in, const AgentSoA& a, const double* xy,
+                               hipStream_t stream);
+
 // ---- Morton re-sort ------------------------------------------------------
 struct SortScratch {
   uint32_t* key;       // per slot: global bin

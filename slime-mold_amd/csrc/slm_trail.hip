@@ -314,6 +314,37 @@ __global__ void encode_trail_kernel(const float* in, uint64_t cells, void* t) {
       static_cast<typename Codec<S>::elem>(Codec<S>::encode(in[i]));
 }
 
+// ---- display colour map (reference src/display.wgsl:29-87) ---------------
+// One thread per texture pixel.  The fit (scale, offsets) is uniform and
+// computed on the host with the shader's f32 operations; each pixel then maps
+// to a cell by f32 division and truncation, looks its clamped intensity up in
+// the 768-byte R[256] G[256] B[256] table, and stores RGBA bytes (alpha 255).
+// The shader stores f32(byte) / 255 to an rgba8unorm texture, which converts
+// back to the same byte, so the bytes are the table's.  Pixels outside the
+// fitted grid are black.
+template <int S>
+__global__ __launch_bounds__(256) void render_kernel(RenderArgs a, LutBytes lut) {
+  __shared__ uint32_t s_lut[192];
+  if (threadIdx.x < 192) s_lut[threadIdx.x] = lut.word[threadIdx.x];
+  __syncthreads();
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256u + threadIdx.x;
+  if (i >= static_cast<uint64_t>(a.tex_w) * a.tex_h) return;
+  uint32_t px = static_cast<uint32_t>(i % a.tex_w), py = static_cast<uint32_t>(i / a.tex_w);
+  float fx = (static_cast<float>(px) - a.off_x) / a.scale;
+  float fy = (static_cast<float>(py) - a.off_y) / a.scale;
+  uint32_t rgba = 0xff000000u;
+  if (fx >= 0.0f && fx < a.sim_w && fy >= 0.0f && fy < a.sim_h) {
+    uint64_t idx = static_cast<uint64_t>(static_cast<int>(fy)) * a.width +
+                   static_cast<uint64_t>(static_cast<int>(fx));
+    float t = Codec<S>::value(static_cast<const typename Codec<S>::elem*>(a.trail)[idx]);
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    uint32_t k = static_cast<uint32_t>(t * 255.0f);
+    auto byte = [&](uint32_t b) { return (s_lut[b >> 2] >> ((b & 3u) * 8u)) & 0xffu; };
+    rgba |= byte(k) | (byte(k + 256u) << 8) | (byte(k + 512u) << 16);
+  }
+  a.out[i] = rgba;
+}
+
 // partials[inst * kMassBlocks + b]: f64 sum of a fixed stride of inst's cells.
 template <int S>
 __global__ __launch_bounds__(256) void mass_partial_kernel(const void* t, const Instance* insts,
@@ -439,6 +470,42 @@ void launch_encode_trail(const float* in, int storage, uint64_t cells, void* t,
     encode_trail_kernel<SLM_STORAGE_F16><<<nb, 256, 0, stream>>>(in, cells, t);
   else
     encode_trail_kernel<SLM_STORAGE_F32><<<nb, 256, 0, stream>>>(in, cells, t);
+}
+
+void render_fit(uint32_t width, uint32_t height, uint32_t tex_w, uint32_t tex_h, RenderArgs& a) {
+  // src/display.wgsl:48-72, in f32 as the shader evaluates it
+  float sim_w = static_cast<float>(width), sim_h = static_cast<float>(height);
+  float tw = static_cast<float>(tex_w), th = static_cast<float>(tex_h);
+  float sim_aspect = sim_w / sim_h;
+  float tex_aspect = tw / th;
+  float scale, off_x = 0.0f, off_y = 0.0f;
+  if (tex_aspect > sim_aspect) {
+    scale = th / sim_h;
+    off_x = (tw - sim_w * scale) * 0.5f;
+  } else {
+    scale = tw / sim_w;
+    off_y = (th - sim_h * scale) * 0.5f;
+  }
+  a.width = width;
+  a.sim_w = sim_w;
+  a.sim_h = sim_h;
+  a.tex_w = tex_w;
+  a.tex_h = tex_h;
+  a.scale = scale;
+  a.off_x = off_x;
+  a.off_y = off_y;
+}
+
+void launch_render(const RenderArgs& a, const LutBytes& lut, int storage, hipStream_t stream) {
+  uint64_t px = static_cast<uint64_t>(a.tex_w) * a.tex_h;
+  if (px == 0) return;
+  uint32_t nb = blocks_for(px, 256);
+  if (storage == SLM_STORAGE_BF16)
+    render_kernel<SLM_STORAGE_BF16><<<nb, 256, 0, stream>>>(a, lut);
+  else if (storage == SLM_STORAGE_F16)
+    render_kernel<SLM_STORAGE_F16><<<nb, 256, 0, stream>>>(a, lut);
+  else
+    render_kernel<SLM_STORAGE_F32><<<nb, 256, 0, stream>>>(a, lut);
 }
 
 void launch_total_mass(const Pool& pool, const void* t, double* partials, double* out,

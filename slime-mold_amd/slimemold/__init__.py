@@ -7,11 +7,12 @@ compute dispatch.  The compute path is libslimemold_hip.so (HIP, gfx950).
 from ._native import (NativeLibraryMissing, Params, SlimeMoldError, Tuning, default_batch_config,
                       default_config, default_params, device_count, lib)
 from .batch import Batch, InstanceSpec
+from .lut import LutData, LutManager
 from .presets import Preset, PresetManager, init_preset_manager
 from .settings import Settings
 from .simulation import Simulation
 
 __all__ = [
-    "Batch", "InstanceSpec", "default_batch_config", "NativeLibraryMissing", "Params", "Preset", "PresetManager", "Settings", "SlimeMoldError", "Tuning",
+    "Batch", "InstanceSpec", "LutData", "LutManager", "default_batch_config", "NativeLibraryMissing", "Params", "Preset", "PresetManager", "Settings", "SlimeMoldError", "Tuning",
     "Simulation", "default_config", "default_params", "device_count", "init_preset_manager", "lib",
 ]

@@ -194,6 +194,16 @@ SIGNATURES = {
     "slm_batch_write_trail": [_P, ctypes.c_uint32, _FP, ctypes.c_uint64],
     "slm_batch_total_mass": [_P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32],
     "slm_batch_get_info": [_P, ctypes.POINTER(BatchInfo)],
+    "slm_render_rgba8": [_P, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p],
+    "slm_render_rgba8_device": [_P, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_void_p],
+    "slm_batch_render_rgba8": [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                               ctypes.c_uint32, ctypes.c_void_p],
+    "slm_snapshot_size": [_P, ctypes.POINTER(ctypes.c_uint64)],
+    "slm_save_snapshot": [_P, ctypes.c_void_p, ctypes.c_uint64],
+    "slm_load_snapshot": [_P, ctypes.c_void_p, ctypes.c_uint64],
+    "slm_create_from_snapshot": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32,
+                                 ctypes.POINTER(_P)],
     "slm_get_tuning": [_P, ctypes.POINTER(Tuning)],
     "slm_set_tuning": [_P, ctypes.POINTER(Tuning)],
     "slm_autotune": [_P, ctypes.c_uint32, ctypes.POINTER(Tuning)],

@@ -95,6 +95,14 @@ class Batch:
     def synchronize(self) -> None:
         N.check(N.lib().slm_batch_synchronize(self.handle))
 
+    def render(self, k: int, lut, tex_w: int, tex_h: int) -> np.ndarray:
+        """RGBA8 image (tex_h, tex_w, 4) of instance k's trail through a LUT."""
+        from .simulation import _lut_bytes
+        out = np.empty((tex_h, tex_w, 4), np.uint8)
+        N.check(N.lib().slm_batch_render_rgba8(self.handle, k, _lut_bytes(lut), tex_w, tex_h,
+                                               out.ctypes.data))
+        return out
+
     # -- launch tuning ------------------------------------------------------
     def get_tuning(self) -> "N.Tuning":
         t = N.Tuning()

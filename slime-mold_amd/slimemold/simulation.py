@@ -22,6 +22,19 @@ def _fptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
+def _lut_bytes(lut):
+    """A LutData, or 768 bytes / uint8 values, as a ctypes buffer."""
+    if hasattr(lut, "to_bytes"):
+        lut = lut.to_bytes()
+    if isinstance(lut, (bytes, bytearray)):
+        data = bytes(lut)
+    else:
+        data = np.asarray(lut, dtype=np.uint8).tobytes()
+    if len(data) != 768:
+        raise ValueError("a LUT is 768 bytes: R[256] G[256] B[256]")
+    return (ctypes.c_uint8 * 768).from_buffer_copy(data)
+
+
 def _params(p: Union[Settings, "N.Params", None]) -> "N.Params":
     if p is None:
         return N.default_params()
@@ -54,6 +67,22 @@ class Simulation:
         N.check(N.lib().slm_create(ctypes.byref(cfg), ctypes.byref(self._params), ctypes.byref(h)))
         self._h: Optional[ctypes.c_void_p] = h
         self.storage = storage
+
+    @classmethod
+    def from_snapshot(cls, data: Union[bytes, bytearray, str], device: int = 0) -> "Simulation":
+        """A new simulation holding a snapshot's state (bytes, or a file path)."""
+        if isinstance(data, str):
+            with open(data, "rb") as f:
+                data = f.read()
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        h = ctypes.c_void_p()
+        N.check(N.lib().slm_create_from_snapshot(buf, len(data), device, ctypes.byref(h)))
+        sim = cls.__new__(cls)
+        sim._h = h
+        sim._params = N.Params()
+        N.check(N.lib().slm_get_params(h, ctypes.byref(sim._params)))
+        sim.storage = {v: k for k, v in N.STORAGE_NAMES.items()}[sim.info().storage]
+        return sim
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
@@ -118,6 +147,34 @@ class Simulation:
 
     def synchronize(self) -> None:
         N.check(N.lib().slm_synchronize(self.handle))
+
+    # -- display and snapshots --------------------------------------------
+    def render(self, lut, tex_w: int, tex_h: int) -> np.ndarray:
+        """RGBA8 image (tex_h, tex_w, 4) of the trail through a 768-byte LUT
+        (R[256] G[256] B[256]; see slimemold.lut)."""
+        lut = _lut_bytes(lut)
+        out = np.empty((tex_h, tex_w, 4), np.uint8)
+        N.check(N.lib().slm_render_rgba8(self.handle, lut, tex_w, tex_h, out.ctypes.data))
+        return out
+
+    def snapshot(self, path: Optional[str] = None) -> bytes:
+        """The simulation's resumable state (written to `path` if given)."""
+        n = ctypes.c_uint64()
+        N.check(N.lib().slm_snapshot_size(self.handle, ctypes.byref(n)))
+        buf = (ctypes.c_uint8 * n.value)()
+        N.check(N.lib().slm_save_snapshot(self.handle, buf, n.value))
+        data = bytes(buf)
+        if path is not None:
+            with open(path, "wb") as f:
+                f.write(data)
+        return data
+
+    def load_snapshot(self, data: Union[bytes, bytearray, str]) -> None:
+        if isinstance(data, str):
+            with open(data, "rb") as f:
+                data = f.read()
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        N.check(N.lib().slm_load_snapshot(self.handle, buf, len(data)))
 
     # -- launch tuning ------------------------------------------------------
     def get_tuning(self) -> "N.Tuning":

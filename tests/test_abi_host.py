@@ -168,6 +168,11 @@ int main() {
   }
   std::printf("]\n");
   if (m.get_preset("No such preset") != nullptr) return 3;
+  std::vector<uint8_t> raw(768);
+  for (int i = 0; i < 768; i++) raw[i] = static_cast<uint8_t>(i % 256);
+  auto lut = slime_mold::LutData::from_bytes("ramp", raw);
+  lut.reverse();
+  if (lut.bytes()[0] != 255 || lut.bytes()[256] != 255 || lut.bytes()[767] != 0) return 4;
   int code = 0;
   try {  // with no device this must throw, never crash
     slime_mold::Simulation sim(64, 64, 16, 1, slime_mold::Settings{});
@@ -214,3 +219,39 @@ def test_cpp_mirror_compiles_and_matches_reference_presets(slm):
         assert code == -4
     else:
         assert code == 0
+
+
+def test_snapshot_format_is_checked_before_any_device_use(slm):
+    """A snapshot built by an independent encoder of the documented layout
+    passes the library's checks (no device here: -4 only after them); every
+    corruption is rejected as an invalid argument first."""
+    import snapshot_util as su
+    import numpy as np
+
+    L = slm.lib()
+    w, h, n = 8, 5, 3
+    agents = np.array([[1.0, 2.0, 0.5, 30.0], [7.5, 4.9, 3.0, 40.0], [0.0, 0.0, 6.0, 50.0]],
+                      np.float32)
+    trail = np.linspace(0, 1, w * h, dtype=np.float32)
+    good = su.build(w, h, agents, trail, steps=12, reassigns=1, seed=9)
+
+    def create(data):
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        out = ctypes.c_void_p()
+        rc = L.slm_create_from_snapshot(buf, len(data), 0, ctypes.byref(out))
+        if rc == 0:
+            L.slm_destroy(out)
+        return rc, L.slm_last_error().decode()
+
+    rc, msg = create(good)
+    assert rc in (0, -4), msg
+    flipped = bytearray(good)
+    flipped[200] ^= 1
+    assert create(bytes(flipped)) == (-1, "snapshot checksum mismatch")
+    assert create(b"XLMSNAP1" + good[8:])[0] == -1
+    assert create(good[:-8])[0] == -1
+    assert create(su.build(w, h, agents, trail, version=2))[0] == -5
+    bad_trail = su.build(w, h, agents, trail[:-1])  # fewer cells than width*height
+    assert create(bad_trail)[0] == -1
+    assert create(su.build(w, h, agents, trail, flags=4))[0] == -1
+    assert create(su.build(w, h, agents, np.zeros(w * h, np.uint16), storage=3))[0] == -1
