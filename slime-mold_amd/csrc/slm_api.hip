@@ -282,6 +282,7 @@ class Engine {
     p.max_items = max_items_;
     p.resort_interval = cfg.resort_interval;
     p.agents_per_group = cfg.agents_per_group;
+    p.preload_rows = preload_rows_;
     return p;
   }
 
@@ -893,6 +894,7 @@ class Engine {
   double* mass_out_ = nullptr;
   uint32_t total_agents_ = 0, total_units_ = 0, total_bins_ = 0;
   uint32_t total_tiles_ = 0, max_items_ = 0;
+  uint32_t preload_rows_ = 0;
   uint32_t* tile_inst_d_ = nullptr;
   WorkItem* items_d_ = nullptr;
   uint32_t* item_total_d_ = nullptr;
@@ -1093,6 +1095,16 @@ class Engine {
     if (sorting() && bins > kMaxTotalBins)
       return fail(SLM_ERR_INVALID_ARGUMENT, "too many re-sort bins: raise sort_cell_log2");
     total_units_ = static_cast<uint32_t>(units);
+    // whole-tile loads when every strip instance has the same 8- or 16-row tiles
+    preload_rows_ = 0;
+    bool first = true;
+    for (const Instance& in : inst) {
+      if (in.ntx == 0) continue;
+      uint32_t r = in.tile_rows == 8u || in.tile_rows == 16u ? in.tile_rows : 0u;
+      preload_rows_ = first || preload_rows_ == r ? r : 0u;
+      if (!first && preload_rows_ == 0u) break;
+      first = false;
+    }
     total_bins_ = sorting() ? static_cast<uint32_t>(bins) : 0;
     SLM_HIP(hipMemcpy(inst_d_, inst.data(), inst.size() * sizeof(Instance), hipMemcpyHostToDevice));
 

@@ -237,6 +237,86 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
   }
 }
 
+// The same strip unit with its tile height R fixed at compile time and all
+// R + 2 rows (tile plus the halo row above and below) loaded before any is
+// used, so each wave keeps (R + 2) KiB-sized row loads in flight; taller tiles
+// re-read fewer halo rows (R + 2 loads per R rows) without losing the memory
+// parallelism that short tiles get from having more waves.
+template <int S, int M, int R>
+__device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
+                                          typename Codec<S>::elem* out, const Instance& inst,
+                                          const DevParams& p, uint32_t tile, uint32_t lane) {
+  constexpr int V = Codec<S>::kVec;
+  const uint32_t w = inst.width, h = inst.height;
+  uint32_t tx = tile % inst.ntx, ty = tile / inst.ntx;
+  uint32_t c = (tx * 64u + lane) * V;
+  bool valid = c < w;
+  bool need_left = valid && lane == 0u;
+  bool need_right = valid && (lane == 63u || c + V >= w);
+  uint32_t lcol = (c == 0u ? w : c) - 1u;
+  uint32_t rcol = c + V >= w ? c + V - w : c + V;
+  const uint32_t y0 = ty * R;
+  const uint32_t rows = min(static_cast<uint32_t>(R), h - y0);  // uniform
+  RawRow<S> raw[R + 2];
+#pragma unroll
+  for (int r = 0; r < R + 2; r++) {
+    // raw[r] holds row y0 + r - 1, wrapped; rows past the tile's last + 1 are unused
+    if (static_cast<uint32_t>(r) <= rows + 1u) {
+      uint32_t y = y0 + static_cast<uint32_t>(r);
+      y = y == 0u ? h - 1u : y - 1u;
+      if (y >= h) y -= h;
+      issue_row<S>(raw[r], in + static_cast<size_t>(y) * w, c, valid, need_left, need_right,
+                   lcol, rcol);
+    }
+  }
+  float prev[V + 2], cur[V + 2], next[V + 2];
+  finish_row<S>(prev, raw[0], need_left, need_right, p);
+  finish_row<S>(cur, raw[1], need_left, need_right, p);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (static_cast<uint32_t>(r) < rows) {
+      finish_row<S>(next, raw[r + 2], need_left, need_right, p);
+      if (valid) {
+        float o[V];
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+          float sum = 0.0f;
+          if (M != kMixCentre) {
+            sum = prev[k] + prev[k + 1];
+            sum = sum + prev[k + 2];
+            sum = sum + cur[k];
+            sum = sum + cur[k + 1];
+            sum = sum + cur[k + 2];
+            sum = sum + next[k];
+            sum = sum + next[k + 1];
+            sum = sum + next[k + 2];
+          }
+          o[k] = diffuse_out<M>(cur[k + 1], sum, p);
+        }
+        *reinterpret_cast<uint4*>(out + static_cast<size_t>(y0 + r) * w + c) =
+            Codec<S>::pack_values(o);
+      }
+#pragma unroll
+      for (int k = 0; k < V + 2; k++) {
+        prev[k] = cur[k];
+        cur[k] = next[k];
+      }
+    }
+  }
+}
+
+// R = 0: streaming strip units of any height; R = 8 or 16: every strip
+// instance of the launch has R-row tiles, loaded whole.
+template <int S, int M, int R>
+__device__ inline void strip_dispatch(const typename Codec<S>::elem* in,
+                                      typename Codec<S>::elem* out, const Instance& inst,
+                                      const DevParams& p, uint32_t tile, uint32_t lane) {
+  if (R == 0)
+    strip_unit<S, M>(in, out, inst, p, tile, lane);
+  else
+    strip_unit_preload<S, M, (R == 0 ? 8 : R)>(in, out, inst, p, tile, lane);
+}
+
 // One 64-cell unit, one cell per lane, any row length.
 template <int S>
 __device__ inline void cell_unit(const typename Codec<S>::elem* in, typename Codec<S>::elem* out,
@@ -272,7 +352,7 @@ struct TrailArgs {
   int xcd_remap;
 };
 
-template <int S>
+template <int S, int R>
 __global__ __launch_bounds__(kTrailBlock) void trail_kernel(TrailArgs a) {
   using elem = typename Codec<S>::elem;
   uint32_t blk = a.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -290,11 +370,11 @@ __global__ __launch_bounds__(kTrailBlock) void trail_kernel(TrailArgs a) {
   if (inst.ntx == 0)
     cell_unit<S>(in, out, inst, p, unit, lane);
   else if (p.rate == 1.0f)
-    strip_unit<S, kMixAverage>(in, out, inst, p, unit, lane);
+    strip_dispatch<S, kMixAverage, R>(in, out, inst, p, unit, lane);
   else if (p.rate == 0.0f)
-    strip_unit<S, kMixCentre>(in, out, inst, p, unit, lane);
+    strip_dispatch<S, kMixCentre, R>(in, out, inst, p, unit, lane);
   else
-    strip_unit<S, kMixGeneral>(in, out, inst, p, unit, lane);
+    strip_dispatch<S, kMixGeneral, R>(in, out, inst, p, unit, lane);
 }
 
 __global__ void set_params_kernel(DevParams* dst, DevParams src) { *dst = src; }
@@ -387,7 +467,12 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
   TrailArgs a{in, out, pool.inst, pool.params, pool.unit_block_inst, pool.count,
               pool.total_units, pool.xcd_remap};
   uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
-  trail_kernel<S><<<nb, kTrailBlock, 0, stream>>>(a);
+  if (pool.preload_rows == 8)
+    trail_kernel<S, 8><<<nb, kTrailBlock, 0, stream>>>(a);
+  else if (pool.preload_rows == 16)
+    trail_kernel<S, 16><<<nb, kTrailBlock, 0, stream>>>(a);
+  else
+    trail_kernel<S, 0><<<nb, kTrailBlock, 0, stream>>>(a);
 }
 
 }  // namespace

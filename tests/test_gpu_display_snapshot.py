@@ -37,18 +37,33 @@ def test_render_matches_oracle(slm, orc, storage):
                 assert_bitwise(got, want, f"render {w}x{h} -> {tw}x{th}")
 
 
+def hip_runtime():
+    """The HIP runtime the library links against (device memory for the test)."""
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    return hip
+
+
 def test_render_into_device_memory_and_batches(slm, orc):
-    torch = pytest.importorskip("torch")
     lut = slm.lut.gray()
+    hip = hip_runtime()
     with slm.Simulation(640, 360, 8000, seed=4, params=preset(slm, "Snake")) as sim:
         sim.step(5)
-        dev = torch.zeros((720, 1280, 4), dtype=torch.uint8, device="cuda:0")
-        torch.cuda.synchronize()  # the fill runs on torch's stream, the render on the library's
-        N = slm._native
-        N.check(N.lib().slm_render_rgba8_device(
-            sim.handle, slm.simulation._lut_bytes(lut), 1280, 720, ctypes.c_void_p(dev.data_ptr())))
-        sim.synchronize()
-        assert_bitwise(dev.cpu().numpy(), orc.render(sim.read_trail(), lut.to_bytes(), 1280, 720),
+        nbytes = 1280 * 720 * 4
+        dev = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(dev), nbytes) == 0
+        try:
+            N = slm._native
+            N.check(N.lib().slm_render_rgba8_device(sim.handle, slm.simulation._lut_bytes(lut),
+                                                    1280, 720, dev))
+            sim.synchronize()
+            got = np.empty((720, 1280, 4), np.uint8)
+            assert hip.hipMemcpy(got.ctypes.data, dev, nbytes, 2) == 0  # device to host
+        finally:
+            hip.hipFree(dev)
+        assert_bitwise(got, orc.render(sim.read_trail(), lut.to_bytes(), 1280, 720),
                        "device render")
     specs = [slm.InstanceSpec(w, h, 3000, seed=k, params=preset(slm, "Waves"))
              for k, (w, h) in enumerate([(300, 200), (128, 512), (37, 23)])]
