@@ -286,9 +286,11 @@ def main():
     if args.copy_probe_mib:
         return copy_probe(args.copy_probe_mib)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "this library runs one simulation per GPU on one GPU; "
+                                   "--gpus must be 1",
                           "n_gpus": args.gpus}))
-        print("bench.py: only --gpus 1 is supported (single-GPU build)", file=sys.stderr)
+        print("bench.py: only --gpus 1 is supported (the library runs on one GPU; instances "
+              "of a batch would shard across GPUs by cost, which is not built)", file=sys.stderr)
         return 2
     import slimemold as slm
     import slimemold.workloads  # noqa: F401

@@ -2,7 +2,7 @@
  * slm_oracle.h - CPU oracle for the slime-mold step.  TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
- * load this library, and only as the checker (or the timed CPU baseline).  The
+ * load this library, and only to check results (or as the timed CPU baseline).  The
  * product (libslimemold_hip.so) never links, loads or calls it.
  *
  * It restates src/compute.wgsl of the reference (agents :57-145, decay

@@ -3,7 +3,7 @@
  *
  * WGSL leaves the accuracy of sin/cos to the executor (compute.wgsl:80-90 for
  * the sensors, :117 for the jitter hash, :126-127 for the move), so the bits the
- * reference produces depend on the driver.  This build fixes one definition and
+ * reference produces depend on the driver.  This library fixes one definition and
  * compiles it for both the GPU kernels and the CPU test oracle, so that the two
  * agree bit for bit; its accuracy against the true sin/cos is tested separately.
  *

@@ -10,8 +10,10 @@ compared only through their mass).  Reported per checkpoint: max and RMS cell
 error on a window, relative mass error of the whole grid, and the run's total
 mass (f64 on the device).
 
-Usage (writes JSON; the default is the 8192^2 / 2^26-agent configuration):
-    python tests/precision_report.py --out gpurun_out/precision.json
+Usage (writes JSON; the default is the 8192^2 / 2^26-agent configuration;
+--combined reports the mixed-instance bf16 workload per grid-size class):
+    python tests/precision_report.py --out precision.json
+    python tests/precision_report.py --combined --out precision_combined.json
 """
 import argparse
 import json
@@ -89,6 +91,43 @@ def run(slm, orc, side=8192, agents_log2=26, steps=5000, every=500, window=512, 
     return out
 
 
+def run_combined(slm, orc, steps=200, per_class=3, threads=0, log=print, count=1024):
+    """The combined workload (mixed instances + one 16384^2 / 2^28-agent
+    instance, bf16): after `steps` steps, one step's mass error against the
+    FP64 trail pass for up to `per_class` instances of every grid size."""
+    specs = slm.workloads.combined(count=count)
+    classes = {}
+    for k, sp in enumerate(specs):
+        if sp.agent_count > 0:
+            classes.setdefault(sp.width, []).append(k)
+    picked = {side: ks[:per_class] for side, ks in sorted(classes.items())}
+    out = {"steps": steps, "instances": len(specs), "classes": {}}
+    t_start = time.perf_counter()
+    with slm.Batch(specs, storage="bf16", resort_interval=20, graph_chunk=100) as b:
+        b.step(steps)
+        before = {k: b.read_trail(k).astype(np.float64) for ks in picked.values() for k in ks}
+        b.step(1)
+        for side, ks in picked.items():
+            rows = []
+            for k in ks:
+                sp = specs[k]
+                t1 = b.read_trail(k).astype(np.float64)
+                marks = orc.deposit_marks(b.read_agents(k), sp.width, sp.height)
+                ref = orc.trail_pass_f64(before.pop(k), sp.params, marks, nthreads=threads)
+                m_gpu, m_ref = float(t1.sum()), float(ref.sum())
+                rows.append({"instance": k, "agents": sp.agent_count,
+                             "mass_rel_err": abs(m_gpu - m_ref) / m_ref if m_ref else 0.0,
+                             "max_abs": float(np.abs(t1 - ref).max())})
+            out["classes"][f"{side}x{side}"] = {
+                "instances_in_class": len(classes[side]), "sampled": rows,
+                "max_mass_rel_err": max(r["mass_rel_err"] for r in rows),
+                "max_abs": max(r["max_abs"] for r in rows)}
+            log(f"[precision] combined {side}^2: max one-step mass err "
+                f"{out['classes'][f'{side}x{side}']['max_mass_rel_err']:.3e} over {len(rows)}")
+    out["seconds"] = time.perf_counter() - t_start
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--side", type=int, default=8192)
@@ -100,9 +139,23 @@ def main():
     ap.add_argument("--f64-positions", action="store_true")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--out", default="")
+    ap.add_argument("--combined", action="store_true",
+                    help="per-size-class report on the combined bf16 workload instead")
+    ap.add_argument("--combined-steps", type=int, default=200)
     args = ap.parse_args()
     import oracle
     import slimemold
+    import slimemold.workloads  # noqa: F401
+
+    if args.combined:
+        res = run_combined(slimemold, oracle, args.combined_steps, threads=args.threads,
+                           log=lambda m: print(m, flush=True))
+        print(json.dumps({c: {k: v for k, v in r.items() if k != "sampled"}
+                          for c, r in res["classes"].items()}))
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(res, f, indent=1)
+        return
 
     res = run(slimemold, oracle, args.side, args.agents_log2, args.steps, args.every,
               args.window, args.seed, f64_positions=args.f64_positions, threads=args.threads,

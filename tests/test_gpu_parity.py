@@ -1,6 +1,6 @@
 """HIP path vs oracle, bit for bit, through the C ABI (ctypes).
 
-The north-star bar is per-agent position/heading within 1e-6 relative and the
+The parity bar is per-agent position/heading within 1e-6 relative and the
 trail within 1e-5 absolute for the first 16 steps; both sides evaluate the same
 f32 operations in the same order (including sin/cos from slm_trig.h), so the
 tests demand exact equality, which is stricter.
