@@ -105,6 +105,14 @@ typedef struct slm_config {
  * outside the window read global memory, so results are unchanged.  Pays off
  * for short sensor distances and frequent re-sorts. */
 #define SLM_FLAG_TILED_SENSING 0x4u
+/* With BF16 storage: the trail pass stores each cell rounded up with
+ * probability equal to the discarded fraction of its f32 value, instead of to
+ * nearest-even.  The decision compares the 16 discarded bits with 16 bits of a
+ * hash of the cell index and the value (MurmurHash3's 32-bit finaliser), so
+ * runs stay deterministic and reproducible.  Unbiased: removes the drift of a
+ * long round-to-nearest BF16 run's mass.  Other storage: SLM_ERR_UNSUPPORTED.
+ * Values given to slm_write_trail are still rounded to nearest. */
+#define SLM_FLAG_STOCHASTIC_ROUNDING 0x8u
 
 typedef struct slm_info {
   uint32_t width, height;

@@ -9,7 +9,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-STORAGE = {"f32": 0, "bf16": 1, "f16": 2}
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "bf16sr": 3}
 
 
 class OrcParams(ctypes.Structure):
@@ -40,6 +40,8 @@ def lib():
             "orc_sign": (ctypes.c_float, [ctypes.c_float]),
             "orc_f32_to_i32": (ctypes.c_int32, [ctypes.c_float]),
             "orc_round_storage": (ctypes.c_float, [ctypes.c_float, ctypes.c_int]),
+            "orc_sr_hash": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+            "orc_f32_to_bf16_sr": (ctypes.c_uint16, [ctypes.c_float, ctypes.c_uint32]),
             "orc_f32_to_bf16": (ctypes.c_uint16, [ctypes.c_float]),
             "orc_bf16_to_f32": (ctypes.c_float, [ctypes.c_uint16]),
             "orc_f32_to_f16": (ctypes.c_uint16, [ctypes.c_float]),

@@ -126,9 +126,35 @@ float orc_f16_to_f32(uint16_t h) {
 }
 
 float orc_round_storage(float v, int storage) {
-  if (storage == ORC_STORAGE_BF16) return orc_bf16_to_f32(orc_f32_to_bf16(v));
+  if (storage == ORC_STORAGE_BF16 || storage == ORC_STORAGE_BF16_SR)
+    return orc_bf16_to_f32(orc_f32_to_bf16(v));
   if (storage == ORC_STORAGE_F16) return orc_f16_to_f32(orc_f32_to_f16(v));
   return v;
+}
+
+/* Stochastic rounding to bfloat16, deterministic: the 16 discarded bits of a
+ * non-negative finite f32 are compared with 16 pseudo-random bits, the low
+ * half of a hash of the cell index and the value's bits (the 32-bit finaliser
+ * of Appleby's MurmurHash3), by adding them and truncating.  The result rounds
+ * up with probability (discarded bits) / 2^16, so it is unbiased. */
+uint32_t orc_sr_hash(uint32_t cell, uint32_t bits) {
+  uint32_t h = (cell * 0x9e3779b1u) ^ bits;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+uint16_t orc_f32_to_bf16_sr(float v, uint32_t cell) {
+  uint32_t b = f32_bits(v);
+  return (uint16_t)((b + (orc_sr_hash(cell, b) & 0xffffu)) >> 16);
+}
+
+float orc_round_storage_at(float v, int storage, uint32_t cell) {
+  if (storage == ORC_STORAGE_BF16_SR) return orc_bf16_to_f32(orc_f32_to_bf16_sr(v, cell));
+  return orc_round_storage(v, storage);
 }
 
 /* SplitMix64 (Steele, Lea & Flood, OOPSLA 2014): output n (0-based) of the
@@ -359,7 +385,7 @@ void orc_trail_pass(const float* tin, const uint8_t* mark, uint32_t w, uint32_t 
       }
       float avg = sum / count; /* :193 */
       float out = orc_mix(t2[y * w + x], avg, rate); /* :194 */
-      tout[y * w + x] = orc_round_storage(out, storage);
+      tout[y * w + x] = orc_round_storage_at(out, storage, (uint32_t)(y * w + x));
     }
   }
   free(t2);

@@ -48,7 +48,9 @@ typedef struct orc_params {
   float blur_sigma;
 } orc_params;
 
-enum { ORC_STORAGE_F32 = 0, ORC_STORAGE_BF16 = 1, ORC_STORAGE_F16 = 2 };
+/* BF16_SR: bfloat16 storage with deterministic stochastic rounding in the
+ * trail pass (orc_f32_to_bf16_sr); other conversions round to nearest. */
+enum { ORC_STORAGE_F32 = 0, ORC_STORAGE_BF16 = 1, ORC_STORAGE_F16 = 2, ORC_STORAGE_BF16_SR = 3 };
 
 /* ---- primitives (exposed for the known-answer tests) -------------------- */
 void orc_sincos(float x, float* s, float* c);
@@ -62,6 +64,9 @@ float orc_mix(float a, float b, float t);
 float orc_sign(float v);
 int32_t orc_f32_to_i32(float v);
 float orc_round_storage(float v, int storage);
+uint32_t orc_sr_hash(uint32_t cell, uint32_t bits);
+uint16_t orc_f32_to_bf16_sr(float v, uint32_t cell);
+float orc_round_storage_at(float v, int storage, uint32_t cell);
 uint16_t orc_f32_to_bf16(float v);
 float orc_bf16_to_f32(uint16_t h);
 uint16_t orc_f32_to_f16(float v);

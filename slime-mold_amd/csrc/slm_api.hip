@@ -62,8 +62,8 @@ constexpr float kMaxSensorDistance = 4194304.0f;  // keeps sensor taps inside i3
 constexpr uint64_t kMaxTotalBins = 1ull << 30;
 constexpr float kMaxTrailValue = 0x1p100f;        // keeps every 3x3 window sum finite
 constexpr uint32_t kTargetWaves = 8192;           // trail-pass units per pool
-constexpr uint32_t kKnownFlags =
-    SLM_FLAG_NO_XCD_REMAP | SLM_FLAG_F64_POSITIONS | SLM_FLAG_TILED_SENSING;
+constexpr uint32_t kKnownFlags = SLM_FLAG_NO_XCD_REMAP | SLM_FLAG_F64_POSITIONS |
+                                 SLM_FLAG_TILED_SENSING | SLM_FLAG_STOCHASTIC_ROUNDING;
 
 bool finite_params(const slm_params& p) {
   const float* f = &p.decay_factor;
@@ -132,6 +132,8 @@ int check_engine_config(const EngineConfig& c, const uint32_t* reserved, size_t 
   if (c.sort_cell_log2 > 12) return fail(SLM_ERR_INVALID_ARGUMENT, "sort_cell_log2 must be <= 12");
   if (c.graph_chunk > 100000) return fail(SLM_ERR_INVALID_ARGUMENT, "graph_chunk must be <= 100000");
   if (c.flags & ~kKnownFlags) return fail(SLM_ERR_INVALID_ARGUMENT, "unknown flags");
+  if ((c.flags & SLM_FLAG_STOCHASTIC_ROUNDING) && c.storage != SLM_STORAGE_BF16)
+    return fail(SLM_ERR_UNSUPPORTED, "stochastic rounding needs bf16 storage");
   for (size_t i = 0; i < nreserved; i++)
     if (reserved[i]) return fail(SLM_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
   return SLM_OK;
@@ -283,6 +285,7 @@ class Engine {
     p.resort_interval = cfg.resort_interval;
     p.agents_per_group = cfg.agents_per_group;
     p.preload_rows = preload_rows_;
+    p.stochastic_rounding = (cfg.flags & SLM_FLAG_STOCHASTIC_ROUNDING) ? 1 : 0;
     return p;
   }
 

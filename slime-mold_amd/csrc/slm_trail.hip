@@ -103,6 +103,56 @@ struct Codec<SLM_STORAGE_F16> {
   }
 };
 
+// bfloat16 with deterministic stochastic rounding (the trail pass's stores
+// only; reads and other conversions are those of bfloat16): the 16 discarded
+// bits are added to the low half of a hash of the cell index and the value's
+// bits (the 32-bit MurmurHash3 finaliser, Appleby) and the sum truncated, so
+// a value rounds up with probability (discarded bits) / 2^16 - unbiased,
+// where round-to-nearest-even drifts a long run's mass by several percent.
+constexpr int kStorageBf16Sr = 3;
+
+__device__ inline uint32_t sr_hash(uint32_t cell, uint32_t bits) {
+  uint32_t h = (cell * 0x9e3779b1u) ^ bits;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+__device__ inline uint32_t bf16_sr(float v, uint32_t cell) {
+  uint32_t b = __float_as_uint(v);  // v >= 0 and finite
+  return (b + (sr_hash(cell, b) & 0xffffu)) >> 16;
+}
+
+template <>
+struct Codec<kStorageBf16Sr> : Codec<SLM_STORAGE_BF16> {
+  __device__ static uint32_t encode_at(float v, uint32_t cell) { return bf16_sr(v, cell); }
+  __device__ static uint4 pack_values_at(const float (&v)[8], uint32_t cell) {
+    return make_uint4(bf16_sr(v[0], cell) | (bf16_sr(v[1], cell + 1u) << 16),
+                      bf16_sr(v[2], cell + 2u) | (bf16_sr(v[3], cell + 3u) << 16),
+                      bf16_sr(v[4], cell + 4u) | (bf16_sr(v[5], cell + 5u) << 16),
+                      bf16_sr(v[6], cell + 6u) | (bf16_sr(v[7], cell + 7u) << 16));
+  }
+};
+
+// Stores of the trail pass, given the instance-local index of the first cell.
+template <int S>
+__device__ inline uint4 pack_out(const float (&v)[Codec<S>::kVec], uint32_t cell) {
+  if constexpr (S == kStorageBf16Sr)
+    return Codec<S>::pack_values_at(v, cell);
+  else
+    return Codec<S>::pack_values(v);
+}
+template <int S>
+__device__ inline uint32_t encode_out(float v, uint32_t cell) {
+  if constexpr (S == kStorageBf16Sr)
+    return Codec<S>::encode_at(v, cell);
+  else
+    return Codec<S>::encode(v);
+}
+
 // Deposit (S3) and decay of one stored element.
 template <int S>
 __device__ inline float deposit_decay(uint32_t raw, const DevParams& p) {
@@ -225,7 +275,7 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
         }
         o[k] = diffuse_out<M>(cur[k + 1], s, p);
       }
-      *reinterpret_cast<uint4*>(out + static_cast<size_t>(y) * w + c) = Codec<S>::pack_values(o);
+      *reinterpret_cast<uint4*>(out + static_cast<size_t>(y) * w + c) = pack_out<S>(o, y * w + c);
     }
 #pragma unroll
     for (int k = 0; k < V + 2; k++) {
@@ -294,7 +344,7 @@ __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
           o[k] = diffuse_out<M>(cur[k + 1], sum, p);
         }
         *reinterpret_cast<uint4*>(out + static_cast<size_t>(y0 + r) * w + c) =
-            Codec<S>::pack_values(o);
+            pack_out<S>(o, (y0 + r) * w + c);
       }
 #pragma unroll
       for (int k = 0; k < V + 2; k++) {
@@ -339,7 +389,7 @@ __device__ inline void cell_unit(const typename Codec<S>::elem* in, typename Cod
     }
   }
   out[i] = static_cast<typename Codec<S>::elem>(
-      Codec<S>::encode(diffuse_out<kMixGeneral>(centre, s, p)));
+      encode_out<S>(diffuse_out<kMixGeneral>(centre, s, p), static_cast<uint32_t>(i)));
 }
 
 struct TrailArgs {
@@ -518,7 +568,10 @@ void launch_trail(const Pool& pool, const void* in, void* out, hipStream_t strea
   if (pool.total_units == 0) return;
   switch (pool.storage) {
     case SLM_STORAGE_BF16:
-      launch_trail_t<SLM_STORAGE_BF16>(pool, in, out, stream);
+      if (pool.stochastic_rounding)
+        launch_trail_t<kStorageBf16Sr>(pool, in, out, stream);
+      else
+        launch_trail_t<SLM_STORAGE_BF16>(pool, in, out, stream);
       break;
     case SLM_STORAGE_F16:
       launch_trail_t<SLM_STORAGE_F16>(pool, in, out, stream);

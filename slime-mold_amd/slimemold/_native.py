@@ -26,6 +26,7 @@ STORAGE_NAMES = {"f32": STORAGE_F32, "bf16": STORAGE_BF16, "f16": STORAGE_F16}
 FLAG_NO_XCD_REMAP = 0x1
 FLAG_F64_POSITIONS = 0x2
 FLAG_TILED_SENSING = 0x4
+FLAG_STOCHASTIC_ROUNDING = 0x8
 
 
 class NativeLibraryMissing(RuntimeError):

@@ -32,7 +32,10 @@ def run_parity(slm, orc, w, h, n, seed, settings, steps, storage="f32", trail0=N
     """Step the GPU and the oracle `steps` times from the same state.
     Returns (gpu_agents, gpu_trail, oracle_agents, oracle_trail)."""
     print(f"[parity] {w}x{h} n={n} seed={seed} steps={steps} storage={storage} {simkw}", flush=True)
-    with slm.Simulation(w, h, n, seed=seed, params=settings, storage=storage, **simkw) as sim:
+    if storage == "bf16sr":  # bf16 storage, stochastic rounding in the trail pass
+        simkw = dict(simkw, stochastic_rounding=True)
+    with slm.Simulation(w, h, n, seed=seed, params=settings,
+                        storage="bf16" if storage == "bf16sr" else storage, **simkw) as sim:
         if agents0 is not None:
             sim.write_agents(agents0)
         if trail0 is not None:

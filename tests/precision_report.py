@@ -51,7 +51,8 @@ def one_step_error(sim, orc, settings, window, threads):
 
 
 def run(slm, orc, side=8192, agents_log2=26, steps=5000, every=500, window=512, seed=7,
-        storages=("bf16", "f16", "f32"), preset="Default", f64_positions=False, threads=0,
+        storages=("bf16", "bf16sr", "f16", "f32"), preset="Default", f64_positions=False,
+        threads=0,
         resort=20, log=print):
     settings = slm.init_preset_manager().get_preset(preset).settings
     out = {"side": side, "agents": 1 << agents_log2, "steps": steps, "every": every,
@@ -61,8 +62,9 @@ def run(slm, orc, side=8192, agents_log2=26, steps=5000, every=500, window=512, 
         t_start = time.perf_counter()
         rows = []
         with slm.Simulation(side, side, 1 << agents_log2, seed=seed, params=settings,
-                            storage=storage, resort_interval=resort, graph_chunk=100,
-                            f64_positions=f64_positions) as sim:
+                            storage="bf16" if storage == "bf16sr" else storage,
+                            stochastic_rounding=storage == "bf16sr", resort_interval=resort,
+                            graph_chunk=100, f64_positions=f64_positions) as sim:
             done = 0
             for c in range(0, steps + 1, every):
                 if c > done:
@@ -91,7 +93,8 @@ def run(slm, orc, side=8192, agents_log2=26, steps=5000, every=500, window=512, 
     return out
 
 
-def run_combined(slm, orc, steps=200, per_class=3, threads=0, log=print, count=1024):
+def run_combined(slm, orc, steps=200, per_class=3, threads=0, log=print, count=1024,
+                 stochastic_rounding=False):
     """The combined workload (mixed instances + one 16384^2 / 2^28-agent
     instance, bf16): after `steps` steps, one step's mass error against the
     FP64 trail pass for up to `per_class` instances of every grid size."""
@@ -103,7 +106,9 @@ def run_combined(slm, orc, steps=200, per_class=3, threads=0, log=print, count=1
     picked = {side: ks[:per_class] for side, ks in sorted(classes.items())}
     out = {"steps": steps, "instances": len(specs), "classes": {}}
     t_start = time.perf_counter()
-    with slm.Batch(specs, storage="bf16", resort_interval=20, graph_chunk=100) as b:
+    out["stochastic_rounding"] = stochastic_rounding
+    with slm.Batch(specs, storage="bf16", resort_interval=20, graph_chunk=100,
+                   stochastic_rounding=stochastic_rounding) as b:
         b.step(steps)
         before = {k: b.read_trail(k).astype(np.float64) for ks in picked.values() for k in ks}
         b.step(1)
@@ -142,6 +147,8 @@ def main():
     ap.add_argument("--combined", action="store_true",
                     help="per-size-class report on the combined bf16 workload instead")
     ap.add_argument("--combined-steps", type=int, default=200)
+    ap.add_argument("--stochastic-rounding", action="store_true",
+                    help="combined report with stochastic bf16 rounding")
     args = ap.parse_args()
     import oracle
     import slimemold
@@ -149,7 +156,8 @@ def main():
 
     if args.combined:
         res = run_combined(slimemold, oracle, args.combined_steps, threads=args.threads,
-                           log=lambda m: print(m, flush=True))
+                           log=lambda m: print(m, flush=True),
+                           stochastic_rounding=args.stochastic_rounding)
         print(json.dumps({c: {k: v for k, v in r.items() if k != "sampled"}
                           for c, r in res["classes"].items()}))
         if args.out:

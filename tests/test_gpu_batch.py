@@ -27,19 +27,21 @@ def mixed_specs(slm, seed=0):
     return specs
 
 
-@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "bf16", "bf16sr"])
 def test_batch_instances_equal_single_runs_and_oracle(slm, orc, storage):
     specs = mixed_specs(slm)
     steps = 12
     print(f"[batch] {len(specs)} instances storage={storage} steps={steps}", flush=True)
-    with slm.Batch(specs, storage=storage, graph_chunk=5) as b:
+    sr = storage == "bf16sr"  # bf16 storage, stochastic rounding in the trail pass
+    kind = "bf16" if sr else storage
+    with slm.Batch(specs, storage=kind, graph_chunk=5, stochastic_rounding=sr) as b:
         starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
         b.step(steps)
         got = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
         masses = b.total_mass()
     for k, s in enumerate(specs):
         with slm.Simulation(s.width, s.height, s.agent_count, seed=s.seed, params=s.params,
-                            storage=storage, graph_chunk=0) as sim:
+                            storage=kind, graph_chunk=0, stochastic_rounding=sr) as sim:
             assert_bitwise(sim.read_agents(), starts[k][0], f"instance {k} init")
             sim.step(steps)
             assert_bitwise(got[k][0], sim.read_agents(), f"instance {k} agents vs single")

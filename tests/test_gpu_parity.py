@@ -37,7 +37,7 @@ def test_every_preset_16_steps(slm, orc, name):
     assert_bitwise(gt, ot, f"{name} trail")
 
 
-@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("storage", ["bf16", "f16", "bf16sr"])
 @pytest.mark.parametrize("name", ["Default", "Curls"])
 def test_16bit_storage_16_steps(slm, orc, storage, name):
     ga, gt, oa, ot = run_parity(slm, orc, 512, 384, 1 << 15, 5, preset(slm, name), 16,
@@ -48,7 +48,7 @@ def test_16bit_storage_16_steps(slm, orc, storage, name):
 
 @pytest.mark.parametrize("w,h", [(37, 23), (1, 1), (1, 17), (17, 1), (2, 3), (1028, 60),
                                  (1030, 40), (1032, 50), (1600, 900), (260, 2)])
-@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "bf16", "bf16sr"])
 def test_ragged_grids(slm, orc, w, h, storage):
     s = preset(slm, "Waves")
     n = min(4000, w * h * 2)
@@ -208,3 +208,10 @@ def test_16bit_write_rounding_matches_oracle(slm, orc, storage):
         sim.write_trail(t)
         got = sim.read_trail()
     assert_bitwise(got, orc.round_storage(t, storage), f"{storage} rounding")
+
+
+def test_stochastic_rounding_needs_bf16(slm):
+    for storage in ("f32", "f16"):
+        with pytest.raises(slm.SlimeMoldError) as e:
+            slm.Simulation(16, 16, 10, storage=storage, stochastic_rounding=True)
+        assert e.value.code == -5

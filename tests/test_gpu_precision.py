@@ -15,13 +15,17 @@ pytestmark = pytest.mark.gpu
 def test_storage_precision_vs_fp64(slm, orc):
     res = precision_report.run(slm, orc, side=1024, agents_log2=18, steps=300, every=100,
                                window=256, seed=7, log=lambda m: print(m, flush=True))
-    bf, fh, f32 = (res["storages"][s] for s in ("bf16", "f16", "f32"))
+    bf, sr, fh, f32 = (res["storages"][s] for s in ("bf16", "bf16sr", "f16", "f32"))
     assert bf["max_mass_rel_err"] <= 1e-3
     assert fh["max_mass_rel_err"] <= 1e-4
     assert f32["max_mass_rel_err"] <= 1e-6
     assert bf["max_window_abs"] <= 2.0 ** -7
+    # stochastic rounding: each cell within one bf16 ulp (2^-7 at values < 1),
+    # the mass unbiased
+    assert sr["max_window_abs"] <= 2.0 ** -7
+    assert sr["max_mass_rel_err"] <= 1e-4
     assert fh["max_window_abs"] <= 2.0 ** -10
     assert f32["max_window_abs"] <= 1e-6
     # whole runs diverge chaotically, but the macroscopic mass stays close
-    for s in ("bf16", "f16"):
+    for s in ("bf16", "bf16sr", "f16"):
         assert max(res["storages"][s]["run_mass_rel_to_f32"]) < 0.05

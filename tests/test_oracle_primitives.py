@@ -133,3 +133,27 @@ def test_uniform24_resolution_and_range(orc):
     assert us.min() >= 0.0 and us.max() < 1.0
     assert np.all((us * 2 ** 24) == np.floor(us * 2 ** 24))  # multiples of 2^-24
     assert abs(us.mean() - 0.5) < 0.01
+
+
+def test_stochastic_bf16_rounding(orc):
+    """Deterministic stochastic rounding: the hash is MurmurHash3's 32-bit
+    finaliser (fmix32(1) = 0x514e28b7, Appleby's published value); results are
+    the two neighbouring bf16 values; a value rounds up with probability equal
+    to its discarded fraction, so the mean over many cells is unbiased; exact
+    bf16 values are kept."""
+    L = orc.lib()
+    assert L.orc_sr_hash(0, 1) == 0x514E28B7
+    base = np.float32(1.0) + np.float32(0.3 * 2 ** -7)  # 30% of the way to the next bf16
+    bits = int(base.view(np.uint32))
+    lo, hi = bits >> 16, (bits >> 16) + 1
+    got = np.array([L.orc_f32_to_bf16_sr(float(base), c) for c in range(1 << 16)])
+    assert set(np.unique(got)) <= {lo, hi}
+    frac_up = float(np.mean(got == hi))
+    assert abs(frac_up - (bits & 0xFFFF) / 65536.0) < 0.01
+    vals = (got.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    assert abs(vals.mean() - float(base)) < 2e-4 * float(base)
+    exact = np.float32(0.8125)  # a bf16 value
+    assert all(L.orc_f32_to_bf16_sr(float(exact), c) == int(exact.view(np.uint32)) >> 16
+               for c in range(1000))
+    # round to nearest would always round this value down: bias 0.3 ulp
+    assert L.orc_f32_to_bf16(float(base)) == lo
