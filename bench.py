@@ -93,6 +93,8 @@ def parse():
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-storage", default="bf16", choices=["f32", "bf16", "f16"])
+    ap.add_argument("--rounding", default="stochastic", choices=["nearest", "stochastic"],
+                    help="bf16 rounding of the batched, combined and bf16 diffuse legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -216,15 +218,16 @@ def autotune(handle, trial_steps, label):
     return d
 
 
-def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps):
+def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps, rounding="nearest"):
     """The trail pass alone on a map larger than the 256 MB Infinity Cache:
     `side`^2 cells and no agents, so every step is one trail-kernel launch;
     graph chunks, wall time of `steps` steps between synchronisations (launch
     gaps included, so a lower bound on the kernel's own bandwidth), after the
     autotuner has picked the tile rows."""
     sb = STORAGE_BYTES[storage]
+    sr = storage == "bf16" and rounding == "stochastic"
     sim = slm.Simulation(side, side, 0, seed=3, params=slm.workloads.preset("Default"),
-                         storage=storage, graph_chunk=graph_chunk)
+                         storage=storage, graph_chunk=graph_chunk, stochastic_rounding=sr)
     tuning = autotune(sim, tune_steps, f"diffuse {side}^2 {storage}")
     sim.step(2 * graph_chunk)
     sim.synchronize()
@@ -239,15 +242,17 @@ def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps):
     return {"side": side, "storage": storage, "map_bytes": cells * sb, "steps": steps,
             "us_per_step": round(dt / steps * 1e6, 2), "achieved_gbps": round(gbps, 1),
             "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
-            "cells_per_s": cells * steps / dt, "trail_rows": tuning["trail_rows"]}
+            "cells_per_s": cells * steps / dt, "trail_rows": tuning["trail_rows"],
+            "rounding": "stochastic" if sr else "nearest"}
 
 
 def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label,
-              tiled=False, tune_steps=0):
+              tiled=False, tune_steps=0, rounding="nearest"):
     sb = STORAGE_BYTES[storage]
     log(f"{label}: {len(specs)} instances storage={storage} creating")
+    sr = storage == "bf16" and rounding == "stochastic"
     b = slm.Batch(specs, storage=storage, resort_interval=resort, graph_chunk=graph_chunk,
-                  sort_cell_log2=cell_log2, tiled_sensing=tiled)
+                  sort_cell_log2=cell_log2, tiled_sensing=tiled, stochastic_rounding=sr)
     info = b.info()
     log(f"{label}: {info.total_agents} agents, {info.total_cells} cells, "
         f"{info.device_bytes / 2**30:.1f} GiB; warmup {warmup}")
@@ -277,6 +282,7 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
         "diffuse_pct_hbm_peak": round(100 * trail_bytes(sb) * n_cells / t_s / 1e9 / HBM_PEAK_GBPS, 2),
         "agents_gbps": round(agent_bytes(sb) * n_agents / a_s / 1e9, 1),
         "graph_chunk": graph_chunk, "tuning": tuning, "measure_steps": M,
+        "rounding": "stochastic" if sr else "nearest",
         "total_mass_sum": float(masses.sum()),
     }
 
@@ -303,7 +309,7 @@ def main():
         specs = slm.workloads.combined()
         res = batch_leg(slm, specs, "bf16", args.steps, args.warmup, args.resort,
                         args.graph_chunk, args.sort_cell_log2, "combined", args.tiled,
-                        args.autotune)
+                        args.autotune, args.rounding)
         out = {
             "metric": METRIC, "value": res["agent_updates_per_s"], "unit": "agent-updates/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -376,12 +382,12 @@ def main():
         out["diffuse_hbm"] = [diffuse_leg(slm, 8192, "f32", args.diffuse_steps, args.graph_chunk,
                                           args.autotune),
                               diffuse_leg(slm, 16384, "bf16", args.diffuse_steps,
-                                          args.graph_chunk, args.autotune)]
+                                          args.graph_chunk, args.autotune, args.rounding)]
     if args.batch_steps > 0:
         specs = slm.workloads.batch_mix(args.batch_count)
         bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.resort,
                          args.graph_chunk, args.sort_cell_log2, "batched", args.tiled,
-                         args.autotune)
+                         args.autotune, args.rounding)
         out["batched"] = bres
         out["batched_vs_plain"] = round(bres["agent_updates_per_s"] / plain["value"], 3)
     if args.cpu_seconds > 0:

@@ -107,8 +107,8 @@ typedef struct slm_config {
 #define SLM_FLAG_TILED_SENSING 0x4u
 /* With BF16 storage: the trail pass stores each cell rounded up with
  * probability equal to the discarded fraction of its f32 value, instead of to
- * nearest-even.  The decision compares the 16 discarded bits with 16 bits of a
- * hash of the cell index and the value (MurmurHash3's 32-bit finaliser), so
+ * nearest-even.  The decision adds the 16 discarded bits to 16 pseudo-random
+ * bits, the top half of ((cell * 0x9e3779b1) ^ value bits) * 0x85ebca6b, so
  * runs stay deterministic and reproducible.  Unbiased: removes the drift of a
  * long round-to-nearest BF16 run's mass.  Other storage: SLM_ERR_UNSUPPORTED.
  * Values given to slm_write_trail are still rounded to nearest. */

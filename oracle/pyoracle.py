@@ -40,7 +40,7 @@ def lib():
             "orc_sign": (ctypes.c_float, [ctypes.c_float]),
             "orc_f32_to_i32": (ctypes.c_int32, [ctypes.c_float]),
             "orc_round_storage": (ctypes.c_float, [ctypes.c_float, ctypes.c_int]),
-            "orc_sr_hash": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+            "orc_sr_bits": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
             "orc_f32_to_bf16_sr": (ctypes.c_uint16, [ctypes.c_float, ctypes.c_uint32]),
             "orc_f32_to_bf16": (ctypes.c_uint16, [ctypes.c_float]),
             "orc_bf16_to_f32": (ctypes.c_float, [ctypes.c_uint16]),

@@ -133,23 +133,17 @@ float orc_round_storage(float v, int storage) {
 }
 
 /* Stochastic rounding to bfloat16, deterministic: the 16 discarded bits of a
- * non-negative finite f32 are compared with 16 pseudo-random bits, the low
- * half of a hash of the cell index and the value's bits (the 32-bit finaliser
- * of Appleby's MurmurHash3), by adding them and truncating.  The result rounds
- * up with probability (discarded bits) / 2^16, so it is unbiased. */
-uint32_t orc_sr_hash(uint32_t cell, uint32_t bits) {
-  uint32_t h = (cell * 0x9e3779b1u) ^ bits;
-  h ^= h >> 16;
-  h *= 0x85ebca6bu;
-  h ^= h >> 13;
-  h *= 0xc2b2ae35u;
-  h ^= h >> 16;
-  return h;
+ * non-negative finite f32 are added to 16 pseudo-random bits, the top half of
+ * ((cell * 0x9e3779b1) ^ bits) * 0x85ebca6b, and the sum truncated.  The
+ * result rounds up with probability (discarded bits) / 2^16, so it is
+ * unbiased. */
+uint32_t orc_sr_bits(uint32_t cell, uint32_t bits) {
+  return (((cell * 0x9e3779b1u) ^ bits) * 0x85ebca6bu) >> 16;
 }
 
 uint16_t orc_f32_to_bf16_sr(float v, uint32_t cell) {
   uint32_t b = f32_bits(v);
-  return (uint16_t)((b + (orc_sr_hash(cell, b) & 0xffffu)) >> 16);
+  return (uint16_t)((b + orc_sr_bits(cell, b)) >> 16);
 }
 
 float orc_round_storage_at(float v, int storage, uint32_t cell) {

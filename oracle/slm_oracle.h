@@ -64,7 +64,7 @@ float orc_mix(float a, float b, float t);
 float orc_sign(float v);
 int32_t orc_f32_to_i32(float v);
 float orc_round_storage(float v, int storage);
-uint32_t orc_sr_hash(uint32_t cell, uint32_t bits);
+uint32_t orc_sr_bits(uint32_t cell, uint32_t bits);
 uint16_t orc_f32_to_bf16_sr(float v, uint32_t cell);
 float orc_round_storage_at(float v, int storage, uint32_t cell);
 uint16_t orc_f32_to_bf16(float v);

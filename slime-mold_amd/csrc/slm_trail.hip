@@ -105,25 +105,18 @@ struct Codec<SLM_STORAGE_F16> {
 
 // bfloat16 with deterministic stochastic rounding (the trail pass's stores
 // only; reads and other conversions are those of bfloat16): the 16 discarded
-// bits are added to the low half of a hash of the cell index and the value's
-// bits (the 32-bit MurmurHash3 finaliser, Appleby) and the sum truncated, so
-// a value rounds up with probability (discarded bits) / 2^16 - unbiased,
-// where round-to-nearest-even drifts a long run's mass by several percent.
+// bits are added to 16 pseudo-random bits and the sum truncated, so a value
+// rounds up with probability (discarded bits) / 2^16 - unbiased, where
+// round-to-nearest-even drifts a long run's mass by several percent.  The
+// random bits are the top half of ((cell * 0x9e3779b1) ^ bits) * 0x85ebca6b:
+// two multiplies (Knuth's and MurmurHash3's odd constants) that mix the cell
+// index and the value, so every cell and step draws differently.
 constexpr int kStorageBf16Sr = 3;
-
-__device__ inline uint32_t sr_hash(uint32_t cell, uint32_t bits) {
-  uint32_t h = (cell * 0x9e3779b1u) ^ bits;
-  h ^= h >> 16;
-  h *= 0x85ebca6bu;
-  h ^= h >> 13;
-  h *= 0xc2b2ae35u;
-  h ^= h >> 16;
-  return h;
-}
 
 __device__ inline uint32_t bf16_sr(float v, uint32_t cell) {
   uint32_t b = __float_as_uint(v);  // v >= 0 and finite
-  return (b + (sr_hash(cell, b) & 0xffffu)) >> 16;
+  uint32_t r = ((cell * 0x9e3779b1u) ^ b) * 0x85ebca6bu;
+  return (b + (r >> 16)) >> 16;
 }
 
 template <>

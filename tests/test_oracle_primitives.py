@@ -136,24 +136,29 @@ def test_uniform24_resolution_and_range(orc):
 
 
 def test_stochastic_bf16_rounding(orc):
-    """Deterministic stochastic rounding: the hash is MurmurHash3's 32-bit
-    finaliser (fmix32(1) = 0x514e28b7, Appleby's published value); results are
-    the two neighbouring bf16 values; a value rounds up with probability equal
-    to its discarded fraction, so the mean over many cells is unbiased; exact
-    bf16 values are kept."""
+    """Deterministic stochastic rounding: results are the two neighbouring bf16
+    values; over many cells the fraction rounded up equals the discarded
+    fraction of the value (so the mean is unbiased, where round-to-nearest is
+    biased by the whole discarded part), for several fractions; the random
+    bits spread over [0, 2^16); exact bf16 values are kept."""
     L = orc.lib()
-    assert L.orc_sr_hash(0, 1) == 0x514E28B7
-    base = np.float32(1.0) + np.float32(0.3 * 2 ** -7)  # 30% of the way to the next bf16
-    bits = int(base.view(np.uint32))
-    lo, hi = bits >> 16, (bits >> 16) + 1
-    got = np.array([L.orc_f32_to_bf16_sr(float(base), c) for c in range(1 << 16)])
-    assert set(np.unique(got)) <= {lo, hi}
-    frac_up = float(np.mean(got == hi))
-    assert abs(frac_up - (bits & 0xFFFF) / 65536.0) < 0.01
-    vals = (got.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
-    assert abs(vals.mean() - float(base)) < 2e-4 * float(base)
+    cells = np.arange(1 << 16)
+    for frac in (0.05, 0.3, 0.5, 0.77, 0.99):
+        base = np.float32(1.0) + np.float32(frac * 2 ** -7)
+        bits = int(base.view(np.uint32))
+        lo, hi = bits >> 16, (bits >> 16) + 1
+        got = np.array([L.orc_f32_to_bf16_sr(float(base), int(c)) for c in cells])
+        assert set(np.unique(got)) <= {lo, hi}
+        want_up = (bits & 0xFFFF) / 65536.0
+        assert abs(float(np.mean(got == hi)) - want_up) < 0.01, frac
+        vals = (got.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        assert abs(vals.mean() - float(base)) < 2e-5 * float(base), frac
+    r = np.array([L.orc_sr_bits(int(c), 0x3F800000) for c in cells[:4096]])
+    assert r.min() >= 0 and r.max() < 65536
+    hist = np.histogram(r, bins=16, range=(0, 65536))[0]
+    assert hist.min() > 4096 / 16 * 0.7 and hist.max() < 4096 / 16 * 1.3
     exact = np.float32(0.8125)  # a bf16 value
     assert all(L.orc_f32_to_bf16_sr(float(exact), c) == int(exact.view(np.uint32)) >> 16
                for c in range(1000))
-    # round to nearest would always round this value down: bias 0.3 ulp
-    assert L.orc_f32_to_bf16(float(base)) == lo
+    base = np.float32(1.0) + np.float32(0.3 * 2 ** -7)
+    assert L.orc_f32_to_bf16(float(base)) == int(base.view(np.uint32)) >> 16  # nearest: down
