@@ -1098,12 +1098,12 @@ class Engine {
     if (sorting() && bins > kMaxTotalBins)
       return fail(SLM_ERR_INVALID_ARGUMENT, "too many re-sort bins: raise sort_cell_log2");
     total_units_ = static_cast<uint32_t>(units);
-    // whole-tile loads when every strip instance has the same 8- or 16-row tiles
+    // whole-tile loads when every strip instance has the same 4-, 8- or 16-row tiles
     preload_rows_ = 0;
     bool first = true;
     for (const Instance& in : inst) {
       if (in.ntx == 0) continue;
-      uint32_t r = in.tile_rows == 8u || in.tile_rows == 16u ? in.tile_rows : 0u;
+      uint32_t r = in.tile_rows == 4u || in.tile_rows == 8u || in.tile_rows == 16u ? in.tile_rows : 0u;
       preload_rows_ = first || preload_rows_ == r ? r : 0u;
       if (!first && preload_rows_ == 0u) break;
       first = false;

@@ -90,7 +90,7 @@ struct Pool {
   uint32_t max_items;
   uint32_t resort_interval;    // steps agents may drift from their tile between re-sorts
   uint32_t agents_per_group;   // agents per workgroup of the global agent pass (k * kAgentBlock)
-  uint32_t preload_rows;       // 8 or 16 when every strip instance has tiles of that height, else 0
+  uint32_t preload_rows;       // 4, 8 or 16 when every strip instance has tiles of that height, else 0
   int stochastic_rounding;     // bf16 trail-pass stores round stochastically (deterministic hash)
 };
 

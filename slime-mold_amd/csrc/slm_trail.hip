@@ -348,7 +348,7 @@ __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
   }
 }
 
-// R = 0: streaming strip units of any height; R = 8 or 16: every strip
+// R = 0: streaming strip units of any height; R = 4, 8 or 16: every strip
 // instance of the launch has R-row tiles, loaded whole.
 template <int S, int M, int R>
 __device__ inline void strip_dispatch(const typename Codec<S>::elem* in,
@@ -510,7 +510,9 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
   TrailArgs a{in, out, pool.inst, pool.params, pool.unit_block_inst, pool.count,
               pool.total_units, pool.xcd_remap};
   uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
-  if (pool.preload_rows == 8)
+  if (pool.preload_rows == 4)
+    trail_kernel<S, 4><<<nb, kTrailBlock, 0, stream>>>(a);
+  else if (pool.preload_rows == 8)
     trail_kernel<S, 8><<<nb, kTrailBlock, 0, stream>>>(a);
   else if (pool.preload_rows == 16)
     trail_kernel<S, 16><<<nb, kTrailBlock, 0, stream>>>(a);
