@@ -215,7 +215,12 @@ __device__ inline void finish_row(float (&t)[Codec<S>::kVec + 2], const RawRow<S
   t[V + 1] = need_right ? deposit_decay<S>(r.right, p) : from_right;
 }
 
-// One strip unit: 64*V columns x tile_rows rows of instance `in`.
+// One strip unit: 64*V columns x tile_rows rows of instance `in`, streamed
+// with kAhead rows in flight: the rows above and at the current output row
+// are decoded in registers, and a ring of kAhead raw rows holds the loads of
+// the rows below; each slot is refilled as soon as it is consumed.
+constexpr int kAhead = 4;
+
 template <int S, int M>
 __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Codec<S>::elem* out,
                                   const Instance& inst, const DevParams& p, uint32_t tile,
@@ -229,54 +234,63 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
   bool need_right = valid && (lane == 63u || c + V >= w);
   uint32_t lcol = (c == 0u ? w : c) - 1u;
   uint32_t rcol = c + V >= w ? c + V - w : c + V;
-  uint32_t y0 = ty * inst.tile_rows;
-  uint32_t y1 = min(y0 + inst.tile_rows, h);
-  auto row_ptr = [&](uint32_t r) { return in + static_cast<size_t>(r) * w; };
-  auto wrap_up = [&](uint32_t r) { return r == 0u ? h - 1u : r - 1u; };
-  auto wrap_down = [&](uint32_t r) { return r + 1u >= h ? r + 1u - h : r + 1u; };
-
+  const uint32_t y0 = ty * inst.tile_rows;
+  const uint32_t y1 = min(y0 + inst.tile_rows, h);
+  // row r of the unit's range y0 - 1 .. y1, wrapped onto the torus
+  auto row_ptr = [&](uint32_t r) {
+    uint32_t y = r == 0u ? h - 1u : (r - 1u >= h ? r - 1u - h : r - 1u);
+    return in + static_cast<size_t>(y) * w;
+  };  // r counts from y0 - 1 + 1: row_ptr(y + 1) is row y
   float prev[V + 2], cur[V + 2], next[V + 2];
-  RawRow<S> rp, rc, rn;
-  issue_row<S>(rp, row_ptr(wrap_up(y0)), c, valid, need_left, need_right, lcol, rcol);
-  issue_row<S>(rc, row_ptr(y0), c, valid, need_left, need_right, lcol, rcol);
-  uint32_t ynext = wrap_down(y0);
-  issue_row<S>(rn, row_ptr(ynext), c, valid, need_left, need_right, lcol, rcol);
-  finish_row<S>(prev, rp, need_left, need_right, p);
-  finish_row<S>(cur, rc, need_left, need_right, p);
-
-  for (uint32_t y = y0; y < y1; y++) {
-    // row y+1 is in flight; start row y+2 before consuming it
-    RawRow<S> ra;
-    bool more = y + 1u < y1;
-    uint32_t yafter = wrap_down(ynext);
-    if (more) issue_row<S>(ra, row_ptr(yafter), c, valid, need_left, need_right, lcol, rcol);
-    finish_row<S>(next, rn, need_left, need_right, p);
-    if (valid) {
-      float o[V];
+  {
+    RawRow<S> rp, rc;
+    issue_row<S>(rp, row_ptr(y0), c, valid, need_left, need_right, lcol, rcol);
+    issue_row<S>(rc, row_ptr(y0 + 1u), c, valid, need_left, need_right, lcol, rcol);
+    finish_row<S>(prev, rp, need_left, need_right, p);
+    finish_row<S>(cur, rc, need_left, need_right, p);
+  }
+  // slot j holds row y + 1 + j while the block of output rows starting at y runs
+  RawRow<S> ring[kAhead];
 #pragma unroll
-      for (int k = 0; k < V; k++) {
-        float s = 0.0f;
-        if (M != kMixCentre) {
-          s = prev[k] + prev[k + 1];
-          s = s + prev[k + 2];
-          s = s + cur[k];
-          s = s + cur[k + 1];
-          s = s + cur[k + 2];
-          s = s + next[k];
-          s = s + next[k + 1];
-          s = s + next[k + 2];
+  for (int j = 0; j < kAhead; j++)
+    if (y0 + 1u + j <= y1)
+      issue_row<S>(ring[j], row_ptr(y0 + 2u + j), c, valid, need_left, need_right, lcol, rcol);
+  for (uint32_t y = y0; y < y1; y += kAhead) {
+#pragma unroll
+    for (int j = 0; j < kAhead; j++) {
+      const uint32_t yy = y + j;
+      if (yy < y1) {  // uniform
+        finish_row<S>(next, ring[j], need_left, need_right, p);  // row yy + 1
+        if (yy + 1u + kAhead <= y1)
+          issue_row<S>(ring[j], row_ptr(yy + 2u + kAhead), c, valid, need_left, need_right, lcol,
+                       rcol);
+        if (valid) {
+          float o[V];
+#pragma unroll
+          for (int k = 0; k < V; k++) {
+            float sum = 0.0f;
+            if (M != kMixCentre) {
+              sum = prev[k] + prev[k + 1];
+              sum = sum + prev[k + 2];
+              sum = sum + cur[k];
+              sum = sum + cur[k + 1];
+              sum = sum + cur[k + 2];
+              sum = sum + next[k];
+              sum = sum + next[k + 1];
+              sum = sum + next[k + 2];
+            }
+            o[k] = diffuse_out<M>(cur[k + 1], sum, p);
+          }
+          *reinterpret_cast<uint4*>(out + static_cast<size_t>(yy) * w + c) =
+              pack_out<S>(o, yy * w + c);
         }
-        o[k] = diffuse_out<M>(cur[k + 1], s, p);
-      }
-      *reinterpret_cast<uint4*>(out + static_cast<size_t>(y) * w + c) = pack_out<S>(o, y * w + c);
-    }
 #pragma unroll
-    for (int k = 0; k < V + 2; k++) {
-      prev[k] = cur[k];
-      cur[k] = next[k];
+        for (int k = 0; k < V + 2; k++) {
+          prev[k] = cur[k];
+          cur[k] = next[k];
+        }
+      }
     }
-    if (more) rn = ra;
-    ynext = yafter;
   }
 }
 
