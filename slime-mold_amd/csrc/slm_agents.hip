@@ -396,8 +396,12 @@ struct AgentArgs {
   int xcd_remap;
 };
 
+// The tables are separate restrict-qualified arguments: nothing the kernel
+// stores aliases them, so their block-uniform loads become scalar loads.
 template <int S, bool P64>
-__global__ __launch_bounds__(kAgentBlock) void agents_kernel(AgentArgs args) {
+__global__ __launch_bounds__(kAgentBlock) void agents_kernel(
+    AgentArgs args, const Instance* __restrict__ insts, const DevParams* __restrict__ params,
+    const uint32_t* __restrict__ block_inst) {
   uint32_t blk = args.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
   using elem = typename TrailRead<S>::elem;
   if (args.group == kAgentBlock) {  // one agent per thread
@@ -405,11 +409,11 @@ __global__ __launch_bounds__(kAgentBlock) void agents_kernel(AgentArgs args) {
     uint32_t last = min(first + kAgentBlock, args.total) - 1u;
     uint32_t j = first + threadIdx.x;
     // instances overlapping this block, one at a time (block-uniform)
-    for (uint32_t ii = args.block_inst[blk];; ii++) {
-      const Instance in = args.inst[ii];
+    for (uint32_t ii = block_inst[blk];; ii++) {
+      const Instance in = insts[ii];
       uint32_t end = in.agent_off + in.agent_count;
       if (j >= in.agent_off && j < end) {
-        const DevParams p = args.params[ii];
+        const DevParams p = params[ii];
         GlobalTaps<S> taps{static_cast<elem*>(args.trail) + in.trail_off, in.width, in.height};
         agent_step<P64>(args.a, j, in, p, taps);
       }
@@ -421,12 +425,12 @@ __global__ __launch_bounds__(kAgentBlock) void agents_kernel(AgentArgs args) {
   // next agent's state in flight
   uint32_t first = blk * args.group;
   uint32_t stop = min(first + args.group, args.total);
-  for (uint32_t ii = args.block_inst[first / kAgentBlock];; ii++) {
-    const Instance in = args.inst[ii];
+  for (uint32_t ii = block_inst[first / kAgentBlock];; ii++) {
+    const Instance in = insts[ii];
     uint32_t end = in.agent_off + in.agent_count;
     uint32_t b = max(first, in.agent_off), e = min(stop, end);
     if (b < e) {
-      const DevParams p = args.params[ii];
+      const DevParams p = params[ii];
       GlobalTaps<S> taps{static_cast<elem*>(args.trail) + in.trail_off, in.width, in.height};
       agent_loop<P64, kAgentBlock>(args.a, b, e, in, p, taps);
     }
@@ -647,9 +651,11 @@ template <int S>
 void launch_agents_t(const Pool& pool, const AgentArgs& args, hipStream_t stream) {
   uint32_t nb = blocks_for(pool.total_agents, args.group);
   if (pool.pos64)
-    agents_kernel<S, true><<<nb, kAgentBlock, 0, stream>>>(args);
+    agents_kernel<S, true><<<nb, kAgentBlock, 0, stream>>>(args, args.inst, args.params,
+                                                           args.block_inst);
   else
-    agents_kernel<S, false><<<nb, kAgentBlock, 0, stream>>>(args);
+    agents_kernel<S, false><<<nb, kAgentBlock, 0, stream>>>(args, args.inst, args.params,
+                                                            args.block_inst);
 }
 
 }  // namespace
