@@ -409,18 +409,21 @@ struct TrailArgs {
   int xcd_remap;
 };
 
+// Tables as restrict arguments: their wave-uniform loads become scalar loads.
 template <int S, int R>
-__global__ __launch_bounds__(kTrailBlock) void trail_kernel(TrailArgs a) {
+__global__ __launch_bounds__(kTrailBlock) void trail_kernel(
+    TrailArgs a, const Instance* __restrict__ insts, const DevParams* __restrict__ params,
+    const uint32_t* __restrict__ block_inst) {
   using elem = typename Codec<S>::elem;
   uint32_t blk = a.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
   uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   uint32_t lane = threadIdx.x & 63u;
   uint32_t u = blk * kTrailWavesPerBlock + wave;
   if (u >= a.total_units) return;  // uniform per wave
-  uint32_t ii = a.block_inst[blk];
-  while (u >= a.inst[ii].unit_off + a.inst[ii].units) ii++;
-  const Instance inst = a.inst[ii];
-  const DevParams p = a.params[ii];
+  uint32_t ii = block_inst[blk];
+  while (u >= insts[ii].unit_off + insts[ii].units) ii++;
+  const Instance inst = insts[ii];
+  const DevParams p = params[ii];
   const elem* in = static_cast<const elem*>(a.in) + inst.trail_off;
   elem* out = static_cast<elem*>(a.out) + inst.trail_off;
   uint32_t unit = u - inst.unit_off;
@@ -525,13 +528,13 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
               pool.total_units, pool.xcd_remap};
   uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
   if (pool.preload_rows == 4)
-    trail_kernel<S, 4><<<nb, kTrailBlock, 0, stream>>>(a);
+    trail_kernel<S, 4><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
   else if (pool.preload_rows == 8)
-    trail_kernel<S, 8><<<nb, kTrailBlock, 0, stream>>>(a);
+    trail_kernel<S, 8><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
   else if (pool.preload_rows == 16)
-    trail_kernel<S, 16><<<nb, kTrailBlock, 0, stream>>>(a);
+    trail_kernel<S, 16><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
   else
-    trail_kernel<S, 0><<<nb, kTrailBlock, 0, stream>>>(a);
+    trail_kernel<S, 0><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
 }
 
 }  // namespace
