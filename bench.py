@@ -8,7 +8,9 @@ whole grid.
 
 Batched leg: the 256-instance mix (grids 256^2..2048^2, 2^14..2^22 agents,
 parameters swept over the preset range) with a bf16 trail, all instances
-advanced by one agent launch and one trail launch per step.
+advanced by one agent launch and one trail launch per step.  Separate leg: the
+same instances as 256 separate f32 simulations with the batch's tuning, the
+plain path on the batched workload ("batched_vs_separate").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
 tiles and a re-sort every 20 steps).  Before its warmup the batched (or
@@ -92,6 +94,9 @@ def parse():
     ap.add_argument("--diffuse-steps", type=int, default=500,
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--separate-steps", type=int, default=-1,
+                    help="steps of the batched leg's instances run as separate f32 "
+                         "simulations (-1: as many as the batched leg, 0: skip)")
     ap.add_argument("--batch-storage", default="bf16", choices=["f32", "bf16", "f16"])
     ap.add_argument("--rounding", default="stochastic", choices=["nearest", "stochastic"],
                     help="bf16 rounding of the batched, combined and bf16 diffuse legs")
@@ -287,6 +292,50 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
     }
 
 
+def separate_leg(slm, specs, storage, steps, warmup, graph_chunk, tuning):
+    """The batched leg's instances as separate single simulations (one handle,
+    stream and graph per instance), with the batch's tuning: the plain path on
+    the same workload.  Each round enqueues one graph chunk per simulation;
+    the streams share the device's hardware queues."""
+    fields = ("resort_interval", "sort_cell_log2", "agents_per_group", "trail_rows",
+              "tiled_sensing")
+    log(f"separate: {len(specs)} simulations storage={storage} creating")
+    sims = []
+    try:
+        for sp in specs:
+            sim = slm.Simulation(sp.width, sp.height, sp.agent_count, seed=sp.seed,
+                                 params=sp.params, storage=storage, graph_chunk=graph_chunk)
+            sim.set_tuning(**{f: tuning[f] for f in fields})
+            sims.append(sim)
+        chunk = max(1, graph_chunk)
+
+        def run(n):
+            done = 0
+            while done < n:
+                k = min(chunk, n - done)
+                for sim in sims:
+                    sim.step(k)
+                done += k
+            for sim in sims:
+                sim.synchronize()
+
+        run(warmup)
+        t0 = time.perf_counter()
+        run(steps)
+        elapsed = time.perf_counter() - t0
+        log(f"separate: timed {steps} steps in {elapsed:.3f} s")
+        n_agents = sum(sp.agent_count for sp in specs)
+        n_cells = sum(sp.width * sp.height for sp in specs)
+    finally:
+        for sim in sims:
+            sim.close()
+    return {"instances": len(specs), "storage": storage, "steps": steps, "warmup": warmup,
+            "agent_updates_per_s": n_agents * steps / elapsed,
+            "cell_updates_per_s": n_cells * steps / elapsed,
+            "ms_per_step": elapsed / steps * 1e3,
+            "tuning": {f: tuning[f] for f in fields}}
+
+
 def main():
     args = parse()
     if args.copy_probe_mib:
@@ -390,6 +439,13 @@ def main():
                          args.autotune, args.rounding)
         out["batched"] = bres
         out["batched_vs_plain"] = round(bres["agent_updates_per_s"] / plain["value"], 3)
+        sep_steps = args.batch_steps if args.separate_steps < 0 else args.separate_steps
+        if sep_steps > 0:
+            sres = separate_leg(slm, specs, "f32", sep_steps, 20, args.graph_chunk,
+                                bres["tuning"])
+            out["separate"] = sres
+            out["batched_vs_separate"] = round(
+                bres["agent_updates_per_s"] / sres["agent_updates_per_s"], 3)
     if args.cpu_seconds > 0:
         import oracle
 
