@@ -8,9 +8,10 @@ whole grid.
 
 Batched leg: the 256-instance mix (grids 256^2..2048^2, 2^14..2^22 agents,
 parameters swept over the preset range) with a bf16 trail, all instances
-advanced by one agent launch and one trail launch per step.  Separate leg: the
-same instances as 256 separate f32 simulations with the batch's tuning, the
-plain path on the batched workload ("batched_vs_separate").
+advanced by one agent launch and one trail launch per step.  Separate leg
+(opt-in, --separate-steps): the same instances as 256 separate f32 simulations
+with the batch's tuning, the plain path on the batched workload
+("batched_vs_separate").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
 tiles and a re-sort every 20 steps).  Before its warmup the batched (or
@@ -94,7 +95,7 @@ def parse():
     ap.add_argument("--diffuse-steps", type=int, default=500,
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
-    ap.add_argument("--separate-steps", type=int, default=-1,
+    ap.add_argument("--separate-steps", type=int, default=0,
                     help="steps of the batched leg's instances run as separate f32 "
                          "simulations (-1: as many as the batched leg, 0: skip)")
     ap.add_argument("--batch-storage", default="bf16", choices=["f32", "bf16", "f16"])
