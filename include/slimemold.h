@@ -279,7 +279,16 @@ typedef struct slm_tuning {
                                 grid so the launch has >= 8192 waves */
   uint32_t tiled_sensing;    /* 1 = SLM_FLAG_TILED_SENSING (needs re-sorting) */
   uint32_t trials;           /* autotune: configurations timed; else 0 */
-  uint32_t reserved[2];      /* must be zero */
+  uint32_t instance_groups;  /* 1..4096 (0 reads as 1): between re-sorts the
+                                instances step as this many contiguous groups
+                                of about equal cost (agents + cells/8), dealt
+                                to 4 streams (parallel branches of the
+                                captured graph), so kernels of different
+                                groups overlap and each covers a few
+                                instances.  More groups than instances act as
+                                one group per instance.  Not with
+                                tiled_sensing (SLM_ERR_UNSUPPORTED). */
+  uint32_t reserved;         /* must be zero */
   /* autotune, for the chosen tuning: step_ms = device time per step from the
      first launch to the end of the last (eager launches, gaps included);
      agents/trail/sort_ms = per-step sums of those kernels' own times (sorts
@@ -293,8 +302,11 @@ int slm_get_tuning(const slm_sim* sim, slm_tuning* out);
 int slm_set_tuning(slm_sim* sim, const slm_tuning* tuning);
 /* Times candidate tunings on the simulation's own state and keeps the
  * fastest: the trail tile rows by trail-pass time, then the re-sort interval
- * (0, 5, 10, 20 or 40), the bin size, tiled sensing and agents per workgroup
- * by total step time (coordinate descent).  Every trial starts from a copy of
+ * (0, 5, 10, 20 or 40), the bin size, tiled sensing, agents per workgroup and
+ * (batches) instance groups (powers of 4 up to the instance count, then the
+ * best one's neighbours x2 and /2) by total step time (coordinate descent).
+ * Step times are spans over steps launched as slm_step launches them; kernel
+ * times come from further steps with events around each whole-pool kernel.  Every trial starts from a copy of
  * the current state, warms up 40 steps and is timed over the next
  * `trial_steps` steps rounded up to a multiple of 40, so all candidates see
  * the same stretch of the simulation.  The state the call

@@ -144,7 +144,8 @@ class Tuning(ctypes.Structure):
         ("trail_rows", ctypes.c_uint32),
         ("tiled_sensing", ctypes.c_uint32),
         ("trials", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("instance_groups", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
         ("step_ms", ctypes.c_double),
         ("agents_ms", ctypes.c_double),
         ("trail_ms", ctypes.c_double),

@@ -129,3 +129,62 @@ def test_f64_positions_in_a_batch(slm):
             sim.step(9)
             assert_bitwise(got[k][0], sim.read_agents(), f"instance {k} agents")
             assert_bitwise(got[k][1], sim.read_trail(), f"instance {k} trail")
+
+
+@pytest.mark.parametrize("groups,chunk,resort,storage", [
+    (2, 0, 0, "f32"), (3, 7, 3, "bf16sr"), (4, 10, 4, "f16"), (16, 5, 0, "f32"), (8, 6, 1, "bf16")])
+def test_instance_groups_equal_one_group(slm, orc, groups, chunk, resort, storage):
+    """Instance groups step concurrently on their own streams, forked and joined
+    around every re-sort: eager and captured, with odd stretches between
+    re-sorts (the trail buffers swap parity per step), more groups than
+    instances, and f64 positions in one case."""
+    specs = mixed_specs(slm, seed=3)
+    sr = storage == "bf16sr"
+    kind = "bf16" if sr else storage
+    f64 = groups == 8
+    outs = []
+    for g in (1, groups):
+        print(f"[groups] {g} chunk={chunk} resort={resort} {storage}", flush=True)
+        with slm.Batch(specs, storage=kind, graph_chunk=chunk, resort_interval=resort,
+                       stochastic_rounding=sr, f64_positions=f64) as b:
+            b.set_tuning(instance_groups=g)
+            assert b.get_tuning().instance_groups == g
+            if g == 1:
+                starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+            b.step(13)
+            b.step(chunk + 3 if chunk else 4)
+            outs.append([(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))])
+            masses = b.total_mass()
+    steps = 13 + (chunk + 3 if chunk else 4)
+    for k in range(len(specs)):
+        assert_bitwise(outs[1][k][0], outs[0][k][0], f"instance {k} agents, {groups} groups")
+        assert_bitwise(outs[1][k][1], outs[0][k][1], f"instance {k} trail, {groups} groups")
+        assert masses[k] == pytest.approx(outs[1][k][1].astype(np.float64).sum(), rel=1e-12,
+                                          abs=1e-9)
+    if not f64:
+        for k, s in enumerate(specs):
+            a, t = starts[k]
+            orc.step(a, t, s.params, steps, storage=storage)
+            assert_bitwise(outs[1][k][0], a, f"instance {k} agents vs oracle")
+            assert_bitwise(outs[1][k][1], t, f"instance {k} trail vs oracle")
+
+
+def test_instance_groups_switched_mid_run(slm):
+    specs = mixed_specs(slm, seed=4)
+    with slm.Batch(specs, graph_chunk=4, resort_interval=3) as ref:
+        ref.step(40)
+        want = [(ref.read_agents(k), ref.read_trail(k)) for k in range(len(specs))]
+    with slm.Batch(specs, graph_chunk=4, resort_interval=3) as b:
+        for g, n in [(4, 9), (1, 5), (7, 11), (2, 8), (4, 7)]:
+            b.set_tuning(instance_groups=g)
+            b.step(n)
+        for k in range(len(specs)):
+            assert_bitwise(b.read_agents(k), want[k][0], f"instance {k} agents")
+            assert_bitwise(b.read_trail(k), want[k][1], f"instance {k} trail")
+        # groups and tiled sensing do not combine
+        import ctypes
+        t = b.get_tuning()
+        t.tiled_sensing = 1
+        assert slm.lib().slm_batch_set_tuning(b.handle, ctypes.byref(t)) == -5
+        b.set_tuning(instance_groups=1, tiled_sensing=1)
+        b.step(2)
