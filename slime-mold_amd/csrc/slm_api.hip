@@ -111,6 +111,7 @@ struct EngineConfig {
   uint32_t agents_per_group = kAgentBlock;
   uint32_t trail_rows = 0;  // 0: chosen per instance by plan_trail
   uint32_t instance_groups = 1;
+  uint32_t streaming_stores = 0;
 };
 
 constexpr uint32_t kMaxInstanceGroups = 4096;
@@ -127,7 +128,7 @@ int check_tuning(const slm_tuning& t) {
   if (t.tiled_sensing > 1) return fail(SLM_ERR_INVALID_ARGUMENT, "tiled_sensing must be 0 or 1");
   if (t.instance_groups > kMaxInstanceGroups)
     return fail(SLM_ERR_INVALID_ARGUMENT, "instance_groups must be <= %u", kMaxInstanceGroups);
-  if (t.reserved) return fail(SLM_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
+  if (t.streaming_stores > 1) return fail(SLM_ERR_INVALID_ARGUMENT, "streaming_stores must be 0 or 1");
   if (t.tiled_sensing && t.instance_groups > 1)
     return fail(SLM_ERR_UNSUPPORTED, "instance groups do not combine with tiled sensing");
   return SLM_OK;
@@ -293,6 +294,7 @@ class Engine {
     p.agents_per_group = cfg.agents_per_group;
     p.preload_rows = preload_rows_;
     p.stochastic_rounding = (cfg.flags & SLM_FLAG_STOCHASTIC_ROUNDING) ? 1 : 0;
+    p.streaming_stores = cfg.streaming_stores ? 1 : 0;
     return p;
   }
 
@@ -630,6 +632,7 @@ class Engine {
     t.trail_rows = cfg.trail_rows;
     t.tiled_sensing = (cfg.flags & SLM_FLAG_TILED_SENSING) ? 1u : 0u;
     t.instance_groups = cfg.instance_groups;
+    t.streaming_stores = cfg.streaming_stores;
     return t;
   }
 
@@ -776,6 +779,7 @@ class Engine {
     cfg.agents_per_group = t.agents_per_group;
     cfg.trail_rows = t.trail_rows;
     cfg.instance_groups = t.instance_groups ? t.instance_groups : 1u;
+    cfg.streaming_stores = t.streaming_stores;
     if (t.tiled_sensing)
       cfg.flags |= SLM_FLAG_TILED_SENSING;
     else
@@ -880,6 +884,13 @@ class Engine {
           bt = r;
         }
       }
+    }
+    // 1b. streaming (non-temporal) trail stores, judged by the whole step: they
+    // also change what the next agent pass finds in the caches
+    {
+      slm_tuning t = best;
+      t.streaming_stores ^= 1u;
+      SLM_TRY(take(t));
     }
     // 2. re-sort interval, 3. bin size, 4. tiled sensing, 5. agents per group
     for (uint32_t k : {0u, 5u, 10u, 20u, 40u}) {

@@ -186,6 +186,18 @@ __device__ inline float diffuse_out(float centre, float sum, const DevParams& p)
   return l + r;
 }
 
+// A strip row's 16-byte store, optionally non-temporal (streamed past the
+// caches: faster when the map does not fit the Infinity Cache).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ inline void store_row(void* dst, uint4 v, bool streaming) {
+  if (streaming) {
+    u32x4 x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<u32x4*>(dst));
+  } else {
+    *reinterpret_cast<uint4*>(dst) = v;
+  }
+}
+
 template <int S>
 struct RawRow {
   uint4 v;
@@ -224,7 +236,7 @@ constexpr int kAhead = 4;
 template <int S, int M>
 __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Codec<S>::elem* out,
                                   const Instance& inst, const DevParams& p, uint32_t tile,
-                                  uint32_t lane) {
+                                  uint32_t lane, bool streaming) {
   constexpr int V = Codec<S>::kVec;
   const uint32_t w = inst.width, h = inst.height;
   uint32_t tx = tile % inst.ntx, ty = tile / inst.ntx;
@@ -281,8 +293,8 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
             }
             o[k] = diffuse_out<M>(cur[k + 1], sum, p);
           }
-          *reinterpret_cast<uint4*>(out + static_cast<size_t>(yy) * w + c) =
-              pack_out<S>(o, yy * w + c);
+          store_row(out + static_cast<size_t>(yy) * w + c, pack_out<S>(o, yy * w + c),
+                    streaming);
         }
 #pragma unroll
         for (int k = 0; k < V + 2; k++) {
@@ -302,7 +314,8 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
 template <int S, int M, int R>
 __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
                                           typename Codec<S>::elem* out, const Instance& inst,
-                                          const DevParams& p, uint32_t tile, uint32_t lane) {
+                                          const DevParams& p, uint32_t tile, uint32_t lane,
+                                          bool streaming) {
   constexpr int V = Codec<S>::kVec;
   const uint32_t w = inst.width, h = inst.height;
   uint32_t tx = tile % inst.ntx, ty = tile / inst.ntx;
@@ -350,8 +363,8 @@ __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
           }
           o[k] = diffuse_out<M>(cur[k + 1], sum, p);
         }
-        *reinterpret_cast<uint4*>(out + static_cast<size_t>(y0 + r) * w + c) =
-            pack_out<S>(o, (y0 + r) * w + c);
+        store_row(out + static_cast<size_t>(y0 + r) * w + c, pack_out<S>(o, (y0 + r) * w + c),
+                  streaming);
       }
 #pragma unroll
       for (int k = 0; k < V + 2; k++) {
@@ -367,11 +380,12 @@ __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
 template <int S, int M, int R>
 __device__ inline void strip_dispatch(const typename Codec<S>::elem* in,
                                       typename Codec<S>::elem* out, const Instance& inst,
-                                      const DevParams& p, uint32_t tile, uint32_t lane) {
+                                      const DevParams& p, uint32_t tile, uint32_t lane,
+                                      bool streaming) {
   if (R == 0)
-    strip_unit<S, M>(in, out, inst, p, tile, lane);
+    strip_unit<S, M>(in, out, inst, p, tile, lane, streaming);
   else
-    strip_unit_preload<S, M, (R == 0 ? 8 : R)>(in, out, inst, p, tile, lane);
+    strip_unit_preload<S, M, (R == 0 ? 8 : R)>(in, out, inst, p, tile, lane, streaming);
 }
 
 // One 64-cell unit, one cell per lane, any row length.
@@ -407,6 +421,7 @@ struct TrailArgs {
   const uint32_t* block_inst;
   uint32_t count, total_units;
   int xcd_remap;
+  int streaming;  // non-temporal strip stores
 };
 
 // Tables as restrict arguments: their wave-uniform loads become scalar loads.
@@ -430,11 +445,11 @@ __global__ __launch_bounds__(kTrailBlock) void trail_kernel(
   if (inst.ntx == 0)
     cell_unit<S>(in, out, inst, p, unit, lane);
   else if (p.rate == 1.0f)
-    strip_dispatch<S, kMixAverage, R>(in, out, inst, p, unit, lane);
+    strip_dispatch<S, kMixAverage, R>(in, out, inst, p, unit, lane, a.streaming != 0);
   else if (p.rate == 0.0f)
-    strip_dispatch<S, kMixCentre, R>(in, out, inst, p, unit, lane);
+    strip_dispatch<S, kMixCentre, R>(in, out, inst, p, unit, lane, a.streaming != 0);
   else
-    strip_dispatch<S, kMixGeneral, R>(in, out, inst, p, unit, lane);
+    strip_dispatch<S, kMixGeneral, R>(in, out, inst, p, unit, lane, a.streaming != 0);
 }
 
 __global__ void set_params_kernel(DevParams* dst, DevParams src) { *dst = src; }
@@ -524,8 +539,8 @@ __global__ __launch_bounds__(256) void mass_final_kernel(const double* partials,
 
 template <int S>
 void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t stream) {
-  TrailArgs a{in, out, pool.inst, pool.params, pool.unit_block_inst, pool.count,
-              pool.total_units, pool.xcd_remap};
+  TrailArgs a{in,         out,         pool.inst,      pool.params, pool.unit_block_inst,
+              pool.count, pool.total_units, pool.xcd_remap, pool.streaming_stores};
   uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
   if (pool.preload_rows == 4)
     trail_kernel<S, 4><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);

@@ -145,7 +145,7 @@ class Tuning(ctypes.Structure):
         ("tiled_sensing", ctypes.c_uint32),
         ("trials", ctypes.c_uint32),
         ("instance_groups", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("streaming_stores", ctypes.c_uint32),
         ("step_ms", ctypes.c_double),
         ("agents_ms", ctypes.c_double),
         ("trail_ms", ctypes.c_double),
@@ -153,7 +153,7 @@ class Tuning(ctypes.Structure):
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _P = ctypes.c_void_p
