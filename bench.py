@@ -14,9 +14,10 @@ with the batch's tuning, the plain path on the batched workload
 ("batched_vs_separate").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
-tiles and a re-sort every 20 steps).  Before its warmup the batched (or
-combined) leg lets the library's autotuner choose its tuning (re-sort interval
-and bin size, agents per workgroup, trail tile rows, tiled sensing) on its own
+tiles, a re-sort every 20 steps and non-temporal trail-pass loads and stores).
+Before its warmup the batched (or combined) leg lets the library's autotuner
+choose its tuning (re-sort interval and bin size, agents per workgroup, trail
+tile rows, tiled sensing, instance groups, streaming) on its own
 state, untimed; each leg reports its tuning under "tuning".
 
 Prints one JSON line: value = plain agent-updates/s over the timed region; the
@@ -49,8 +50,8 @@ WRITE_SIZE_SCALE = 1.0
 
 # HBM bytes per launch of the plain leg's dominant kernel at the default
 # workload (4096^2 f32, 2^24 agents, re-sort every 20 steps, 16-row trail
-# tiles): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, recorded
-# in profiles/agents_trail_c1_counters.md.
+# tiles, streaming 3): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per
+# dispatch, recorded in profiles/agents_trail_c1_counters.md.
 MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (179704 * 2 + 274515) * 1024}
 
 
@@ -84,7 +85,9 @@ def parse():
     ap.add_argument("--trail-rows", type=int, default=16,
                     help="plain leg: rows per trail-pass tile (0: chosen from the grid)")
     ap.add_argument("--agents-per-group", type=int, default=256)
-    ap.add_argument("--autotune", type=int, default=20,
+    ap.add_argument("--streaming", type=int, default=3, choices=[0, 1, 2, 3],
+                    help="plain leg: non-temporal trail-pass stores (1), loads (2) or both (3)")
+    ap.add_argument("--autotune", type=int, default=80,
                     help="autotune trial steps before the batched/combined leg's warmup (0: off)")
     ap.add_argument("--autotune-plain", action="store_true",
                     help="also autotune the plain leg (otherwise it runs the flags' tuning)")
@@ -179,7 +182,8 @@ def plain_leg(slm, args):
     sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
                          resort_interval=args.resort, graph_chunk=args.graph_chunk,
                          sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled)
-    sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group)
+    sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
+                   streaming=args.streaming)
     log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
         f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}")
     state0 = (sim.read_agents(), sim.read_trail()) if args.cpu_seconds > 0 else None
@@ -378,7 +382,8 @@ def main():
     dominant = dict(dominant)
     dominant["traffic"] = MEASURED_TRAFFIC.get(
         (dominant["kernel"], W, H, N, args.storage)
-        if (args.resort, args.trail_rows, args.agents_per_group) == (20, 16, 256) else None)
+        if (args.resort, args.trail_rows, args.agents_per_group, args.streaming) == (20, 16, 256, 3)
+        else None)
     out = {
         "metric": METRIC,
         "value": plain["value"],

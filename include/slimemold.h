@@ -288,10 +288,12 @@ typedef struct slm_tuning {
                                 instances.  More groups than instances act as
                                 one group per instance.  Not with
                                 tiled_sensing (SLM_ERR_UNSUPPORTED). */
-  uint32_t streaming_stores; /* 1 = the trail pass writes rows with
-                                non-temporal stores (streamed past the caches):
-                                faster for maps beyond the 256 MB Infinity
-                                Cache, e.g. 8192^2 f32 */
+  uint32_t streaming;        /* trail-pass row accesses streamed past the
+                                caches: bit 0 non-temporal stores, bit 1
+                                non-temporal loads (0..3).  Pays for maps
+                                beyond the 256 MB Infinity Cache in f32
+                                (8192^2: 3 gives 75% of HBM peak against 72%
+                                with 0); costs on 16384^2 bf16 */
   /* autotune, for the chosen tuning: step_ms = device time per step from the
      first launch to the end of the last (eager launches, gaps included);
      agents/trail/sort_ms = per-step sums of those kernels' own times (sorts
@@ -304,7 +306,7 @@ int slm_get_tuning(const slm_sim* sim, slm_tuning* out);
  * are kept; captured graphs are rebuilt on the next slm_step. */
 int slm_set_tuning(slm_sim* sim, const slm_tuning* tuning);
 /* Times candidate tunings on the simulation's own state and keeps the
- * fastest: the trail tile rows by trail-pass time, streaming stores, then the
+ * fastest: the trail tile rows by trail-pass time, streaming (0..3), then the
  * re-sort interval
  * (0, 5, 10, 20 or 40), the bin size, tiled sensing, agents per workgroup and
  * (batches) instance groups (powers of 4 up to the instance count, then the

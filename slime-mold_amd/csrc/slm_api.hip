@@ -111,7 +111,7 @@ struct EngineConfig {
   uint32_t agents_per_group = kAgentBlock;
   uint32_t trail_rows = 0;  // 0: chosen per instance by plan_trail
   uint32_t instance_groups = 1;
-  uint32_t streaming_stores = 0;
+  uint32_t streaming = 0;
 };
 
 constexpr uint32_t kMaxInstanceGroups = 4096;
@@ -128,7 +128,7 @@ int check_tuning(const slm_tuning& t) {
   if (t.tiled_sensing > 1) return fail(SLM_ERR_INVALID_ARGUMENT, "tiled_sensing must be 0 or 1");
   if (t.instance_groups > kMaxInstanceGroups)
     return fail(SLM_ERR_INVALID_ARGUMENT, "instance_groups must be <= %u", kMaxInstanceGroups);
-  if (t.streaming_stores > 1) return fail(SLM_ERR_INVALID_ARGUMENT, "streaming_stores must be 0 or 1");
+  if (t.streaming > 3) return fail(SLM_ERR_INVALID_ARGUMENT, "streaming must be 0..3");
   if (t.tiled_sensing && t.instance_groups > 1)
     return fail(SLM_ERR_UNSUPPORTED, "instance groups do not combine with tiled sensing");
   return SLM_OK;
@@ -294,7 +294,7 @@ class Engine {
     p.agents_per_group = cfg.agents_per_group;
     p.preload_rows = preload_rows_;
     p.stochastic_rounding = (cfg.flags & SLM_FLAG_STOCHASTIC_ROUNDING) ? 1 : 0;
-    p.streaming_stores = cfg.streaming_stores ? 1 : 0;
+    p.streaming = static_cast<int>(cfg.streaming);
     return p;
   }
 
@@ -632,7 +632,7 @@ class Engine {
     t.trail_rows = cfg.trail_rows;
     t.tiled_sensing = (cfg.flags & SLM_FLAG_TILED_SENSING) ? 1u : 0u;
     t.instance_groups = cfg.instance_groups;
-    t.streaming_stores = cfg.streaming_stores;
+    t.streaming = cfg.streaming;
     return t;
   }
 
@@ -779,7 +779,7 @@ class Engine {
     cfg.agents_per_group = t.agents_per_group;
     cfg.trail_rows = t.trail_rows;
     cfg.instance_groups = t.instance_groups ? t.instance_groups : 1u;
-    cfg.streaming_stores = t.streaming_stores;
+    cfg.streaming = t.streaming;
     if (t.tiled_sensing)
       cfg.flags |= SLM_FLAG_TILED_SENSING;
     else
@@ -885,13 +885,6 @@ class Engine {
         }
       }
     }
-    // 1b. streaming (non-temporal) trail stores, judged by the whole step: they
-    // also change what the next agent pass finds in the caches
-    {
-      slm_tuning t = best;
-      t.streaming_stores ^= 1u;
-      SLM_TRY(take(t));
-    }
     // 2. re-sort interval, 3. bin size, 4. tiled sensing, 5. agents per group
     for (uint32_t k : {0u, 5u, 10u, 20u, 40u}) {
       if (k == best.resort_interval) continue;
@@ -921,22 +914,43 @@ class Engine {
     }
     // 6. instance groups (batches): powers of 4 up to the instance count, then
     // the neighbours (x2, /2) of the best
-    if (!best.tiled_sensing && count() > 1) {
+    std::vector<uint32_t> tried_groups;
+    auto try_groups = [&](uint32_t g) -> int {
       const uint32_t cap = std::min<uint32_t>(count(), kMaxInstanceGroups);
-      std::vector<uint32_t> tried{best.instance_groups ? best.instance_groups : 1u};
-      auto try_groups = [&](uint32_t g) -> int {
-        if (g < 1 || g > cap) return SLM_OK;
-        for (uint32_t x : tried)
-          if (x == g) return SLM_OK;
-        tried.push_back(g);
-        slm_tuning t = best;
-        t.instance_groups = g;
-        return take(t);
-      };
-      for (uint32_t g = 1; g <= cap; g *= 4) SLM_TRY(try_groups(g));
+      if (best.tiled_sensing || g < 1 || g > cap) return SLM_OK;
+      for (uint32_t x : tried_groups)
+        if (x == g) return SLM_OK;
+      tried_groups.push_back(g);
+      slm_tuning t = best;
+      t.instance_groups = g;
+      return take(t);
+    };
+    auto group_neighbours = [&]() -> int {
       const uint32_t b = best.instance_groups ? best.instance_groups : 1u;
       SLM_TRY(try_groups(b * 2));
-      SLM_TRY(try_groups(b / 2));
+      return try_groups(b / 2);
+    };
+    if (count() > 1) {
+      tried_groups.push_back(best.instance_groups ? best.instance_groups : 1u);
+      for (uint32_t g = 1; g <= std::min<uint32_t>(count(), kMaxInstanceGroups); g *= 4)
+        SLM_TRY(try_groups(g));
+      SLM_TRY(group_neighbours());
+    }
+    // 7. streaming (non-temporal) trail-pass loads and stores, judged by the
+    // whole step: they also change what the next agent pass finds in the
+    // caches; then the group count's neighbours again under the chosen mode
+    {
+      const slm_tuning base = best;
+      for (uint32_t m = 0; m < 4; m++) {
+        if (m == base.streaming) continue;
+        slm_tuning t = base;
+        t.streaming = m;
+        SLM_TRY(take(t));
+      }
+      if (count() > 1 && best.streaming != base.streaming) {
+        tried_groups.assign(1, best.instance_groups ? best.instance_groups : 1u);
+        SLM_TRY(group_neighbours());
+      }
     }
     return SLM_OK;
   }

@@ -92,7 +92,7 @@ struct Pool {
   uint32_t agents_per_group;   // agents per workgroup of the global agent pass (k * kAgentBlock)
   uint32_t preload_rows;       // 4, 8 or 16 when every strip instance has tiles of that height, else 0
   int stochastic_rounding;     // bf16 trail-pass stores round stochastically (deterministic hash)
-  int streaming_stores;        // trail-pass row stores non-temporal
+  int streaming;               // trail-pass rows: 1 non-temporal stores, 2 non-temporal loads
 };
 
 constexpr uint32_t kTileLog2 = 6;  // sensing tiles of 64 x 64 cells

@@ -186,9 +186,10 @@ __device__ inline float diffuse_out(float centre, float sum, const DevParams& p)
   return l + r;
 }
 
-// A strip row's 16-byte store, optionally non-temporal (streamed past the
-// caches: faster when the map does not fit the Infinity Cache).
+// A strip row's 16-byte store and load, optionally non-temporal (streamed
+// past the caches: faster when the map does not fit the Infinity Cache).
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kStreamStores = 1, kStreamLoads = 2;  // TrailArgs::streaming bits
 __device__ inline void store_row(void* dst, uint4 v, bool streaming) {
   if (streaming) {
     u32x4 x = {v.x, v.y, v.z, v.w};
@@ -207,8 +208,15 @@ struct RawRow {
 template <int S>
 __device__ inline void issue_row(RawRow<S>& r, const typename Codec<S>::elem* row, uint32_t c,
                                  bool valid, bool need_left, bool need_right, uint32_t lcol,
-                                 uint32_t rcol) {
-  r.v = valid ? *reinterpret_cast<const uint4*>(row + c) : make_uint4(0u, 0u, 0u, 0u);
+                                 uint32_t rcol, bool streaming) {
+  if (!valid) {
+    r.v = make_uint4(0u, 0u, 0u, 0u);
+  } else if (streaming) {
+    u32x4 x = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(row + c));
+    r.v = make_uint4(x.x, x.y, x.z, x.w);
+  } else {
+    r.v = *reinterpret_cast<const uint4*>(row + c);
+  }
   r.left = need_left ? static_cast<uint32_t>(row[lcol]) : 0u;
   r.right = need_right ? static_cast<uint32_t>(row[rcol]) : 0u;
 }
@@ -236,7 +244,8 @@ constexpr int kAhead = 4;
 template <int S, int M>
 __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Codec<S>::elem* out,
                                   const Instance& inst, const DevParams& p, uint32_t tile,
-                                  uint32_t lane, bool streaming) {
+                                  uint32_t lane, int streaming) {
+  const bool nt_load = (streaming & kStreamLoads) != 0;
   constexpr int V = Codec<S>::kVec;
   const uint32_t w = inst.width, h = inst.height;
   uint32_t tx = tile % inst.ntx, ty = tile / inst.ntx;
@@ -256,8 +265,8 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
   float prev[V + 2], cur[V + 2], next[V + 2];
   {
     RawRow<S> rp, rc;
-    issue_row<S>(rp, row_ptr(y0), c, valid, need_left, need_right, lcol, rcol);
-    issue_row<S>(rc, row_ptr(y0 + 1u), c, valid, need_left, need_right, lcol, rcol);
+    issue_row<S>(rp, row_ptr(y0), c, valid, need_left, need_right, lcol, rcol, nt_load);
+    issue_row<S>(rc, row_ptr(y0 + 1u), c, valid, need_left, need_right, lcol, rcol, nt_load);
     finish_row<S>(prev, rp, need_left, need_right, p);
     finish_row<S>(cur, rc, need_left, need_right, p);
   }
@@ -266,7 +275,8 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
 #pragma unroll
   for (int j = 0; j < kAhead; j++)
     if (y0 + 1u + j <= y1)
-      issue_row<S>(ring[j], row_ptr(y0 + 2u + j), c, valid, need_left, need_right, lcol, rcol);
+      issue_row<S>(ring[j], row_ptr(y0 + 2u + j), c, valid, need_left, need_right, lcol, rcol,
+                   nt_load);
   for (uint32_t y = y0; y < y1; y += kAhead) {
 #pragma unroll
     for (int j = 0; j < kAhead; j++) {
@@ -275,7 +285,7 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
         finish_row<S>(next, ring[j], need_left, need_right, p);  // row yy + 1
         if (yy + 1u + kAhead <= y1)
           issue_row<S>(ring[j], row_ptr(yy + 2u + kAhead), c, valid, need_left, need_right, lcol,
-                       rcol);
+                       rcol, nt_load);
         if (valid) {
           float o[V];
 #pragma unroll
@@ -294,7 +304,7 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
             o[k] = diffuse_out<M>(cur[k + 1], sum, p);
           }
           store_row(out + static_cast<size_t>(yy) * w + c, pack_out<S>(o, yy * w + c),
-                    streaming);
+                    (streaming & kStreamStores) != 0);
         }
 #pragma unroll
         for (int k = 0; k < V + 2; k++) {
@@ -315,7 +325,8 @@ template <int S, int M, int R>
 __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
                                           typename Codec<S>::elem* out, const Instance& inst,
                                           const DevParams& p, uint32_t tile, uint32_t lane,
-                                          bool streaming) {
+                                          int streaming) {
+  const bool nt_load = (streaming & kStreamLoads) != 0;
   constexpr int V = Codec<S>::kVec;
   const uint32_t w = inst.width, h = inst.height;
   uint32_t tx = tile % inst.ntx, ty = tile / inst.ntx;
@@ -336,7 +347,7 @@ __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
       y = y == 0u ? h - 1u : y - 1u;
       if (y >= h) y -= h;
       issue_row<S>(raw[r], in + static_cast<size_t>(y) * w, c, valid, need_left, need_right,
-                   lcol, rcol);
+                   lcol, rcol, nt_load);
     }
   }
   float prev[V + 2], cur[V + 2], next[V + 2];
@@ -364,7 +375,7 @@ __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
           o[k] = diffuse_out<M>(cur[k + 1], sum, p);
         }
         store_row(out + static_cast<size_t>(y0 + r) * w + c, pack_out<S>(o, (y0 + r) * w + c),
-                  streaming);
+                  (streaming & kStreamStores) != 0);
       }
 #pragma unroll
       for (int k = 0; k < V + 2; k++) {
@@ -381,7 +392,7 @@ template <int S, int M, int R>
 __device__ inline void strip_dispatch(const typename Codec<S>::elem* in,
                                       typename Codec<S>::elem* out, const Instance& inst,
                                       const DevParams& p, uint32_t tile, uint32_t lane,
-                                      bool streaming) {
+                                      int streaming) {
   if (R == 0)
     strip_unit<S, M>(in, out, inst, p, tile, lane, streaming);
   else
@@ -421,7 +432,7 @@ struct TrailArgs {
   const uint32_t* block_inst;
   uint32_t count, total_units;
   int xcd_remap;
-  int streaming;  // non-temporal strip stores
+  int streaming;  // kStreamStores | kStreamLoads: non-temporal strip row stores / loads
 };
 
 // Tables as restrict arguments: their wave-uniform loads become scalar loads.
@@ -445,11 +456,11 @@ __global__ __launch_bounds__(kTrailBlock) void trail_kernel(
   if (inst.ntx == 0)
     cell_unit<S>(in, out, inst, p, unit, lane);
   else if (p.rate == 1.0f)
-    strip_dispatch<S, kMixAverage, R>(in, out, inst, p, unit, lane, a.streaming != 0);
+    strip_dispatch<S, kMixAverage, R>(in, out, inst, p, unit, lane, a.streaming);
   else if (p.rate == 0.0f)
-    strip_dispatch<S, kMixCentre, R>(in, out, inst, p, unit, lane, a.streaming != 0);
+    strip_dispatch<S, kMixCentre, R>(in, out, inst, p, unit, lane, a.streaming);
   else
-    strip_dispatch<S, kMixGeneral, R>(in, out, inst, p, unit, lane, a.streaming != 0);
+    strip_dispatch<S, kMixGeneral, R>(in, out, inst, p, unit, lane, a.streaming);
 }
 
 __global__ void set_params_kernel(DevParams* dst, DevParams src) { *dst = src; }
@@ -540,7 +551,7 @@ __global__ __launch_bounds__(256) void mass_final_kernel(const double* partials,
 template <int S>
 void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t stream) {
   TrailArgs a{in,         out,         pool.inst,      pool.params, pool.unit_block_inst,
-              pool.count, pool.total_units, pool.xcd_remap, pool.streaming_stores};
+              pool.count, pool.total_units, pool.xcd_remap, pool.streaming};
   uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
   if (pool.preload_rows == 4)
     trail_kernel<S, 4><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);

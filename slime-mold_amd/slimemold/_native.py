@@ -145,7 +145,7 @@ class Tuning(ctypes.Structure):
         ("tiled_sensing", ctypes.c_uint32),
         ("trials", ctypes.c_uint32),
         ("instance_groups", ctypes.c_uint32),
-        ("streaming_stores", ctypes.c_uint32),
+        ("streaming", ctypes.c_uint32),
         ("step_ms", ctypes.c_double),
         ("agents_ms", ctypes.c_double),
         ("trail_ms", ctypes.c_double),

@@ -5,7 +5,8 @@ included), an agent count (0 to 2 per cell), a preset with every swept
 parameter perturbed (diffusion rates outside [0, 1], zero or large sensor
 distances, jitter on or off), a storage format, optionally a random starting
 trail, and a launch tuning (re-sort interval and bin size, tiled sensing,
-agents per workgroup, trail tile rows, streaming stores, graph chunk length;
+agents per workgroup, trail tile rows, streaming loads and stores, graph
+chunk length;
 for batches the instance groups).  A tuning never changes a result bit, so
 every case must equal the oracle exactly.  The case is printed before it runs.
 """
@@ -48,7 +49,7 @@ def random_tuning(rng, batch):
              sort_cell_log2=int(rng.integers(2, 7)),
              agents_per_group=int(rng.choice([256, 512, 1024, 2048, 4096])),
              trail_rows=int(rng.choice([0, 4, 8, 16, 32, 64])),
-             streaming_stores=int(rng.random() < 0.5),
+             streaming=int(rng.integers(0, 4)),
              tiled_sensing=int(resort > 0 and rng.random() < 0.3))
     if batch and not t["tiled_sensing"]:
         t["instance_groups"] = int(rng.choice([1, 2, 3, 8]))

@@ -16,9 +16,10 @@ from parity_util import assert_bitwise, preset
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [
-    dict(agents_per_group=512, streaming_stores=1),
+    dict(agents_per_group=512, streaming=1),
     dict(agents_per_group=4096, trail_rows=4),
-    dict(trail_rows=64, streaming_stores=0),
+    dict(trail_rows=64, streaming=3),
+    dict(trail_rows=8, streaming=2),
     dict(resort_interval=3, sort_cell_log2=3),
     dict(resort_interval=2, sort_cell_log2=5, tiled_sensing=1, agents_per_group=2048),
     dict(resort_interval=0, tiled_sensing=1, trail_rows=8),
@@ -50,7 +51,7 @@ def test_set_tuning_rejects_bad_values(slm):
         for fields in [dict(agents_per_group=300), dict(agents_per_group=8192),
                        dict(agents_per_group=128), dict(trail_rows=3), dict(trail_rows=128),
                        dict(sort_cell_log2=13), dict(tiled_sensing=2),
-                       dict(instance_groups=4097), dict(streaming_stores=2)]:
+                       dict(instance_groups=4097), dict(streaming=4)]:
             with pytest.raises(slm.SlimeMoldError) as e:
                 sim.set_tuning(**fields)
             assert e.value.code == -1, fields
