@@ -52,7 +52,7 @@ WRITE_SIZE_SCALE = 1.0
 # workload (4096^2 f32, 2^24 agents, re-sort every 20 steps, 16-row trail
 # tiles, streaming 3): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per
 # dispatch, recorded in profiles/agents_trail_c1_counters.md.
-MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (179704 * 2 + 274515) * 1024}
+MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (179790 * 2 + 274482) * 1024}
 
 
 def agent_bytes(sb, pos64=False):
