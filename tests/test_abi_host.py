@@ -255,3 +255,39 @@ def test_snapshot_format_is_checked_before_any_device_use(slm):
     assert create(bad_trail)[0] == -1
     assert create(su.build(w, h, agents, trail, flags=4))[0] == -1
     assert create(su.build(w, h, agents, np.zeros(w * h, np.uint16), storage=3))[0] == -1
+
+
+def test_ctypes_mirror_matches_the_c_layout(slm):
+    """Every public struct's size and field offsets, as the C compiler lays them
+    out from include/slimemold.h, equal the Python binding's ctypes mirror."""
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no gcc")
+    N = slm._native
+    structs = {"slm_params": N.Params, "slm_config": N.Config, "slm_info": N.Info,
+               "slm_timing": N.Timing, "slm_instance_desc": N.InstanceDesc,
+               "slm_batch_config": N.BatchConfig, "slm_batch_info": N.BatchInfo,
+               "slm_tuning": N.Tuning}
+    lines = ['#include "slimemold.h"', "#include <stddef.h>", "#include <stdio.h>",
+             "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "layout.c")
+        with open(c, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        subprocess.check_call([cc, "-std=c99", "-Wall", "-Werror", "-I",
+                               os.path.join(ROOT, "include"), c, "-o", os.path.join(d, "layout")])
+        out = subprocess.check_output([os.path.join(d, "layout")]).decode().split("\n")
+    got = {}
+    for line in filter(None, out):
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
