@@ -1299,21 +1299,19 @@ class Engine {
     uint32_t slot_base = 0, agents = 0, unit_base = 0, units = 0;
     uint64_t trail_base = 0;
     uint32_t preload_rows = 0;
-    Instance* inst_d = nullptr;
-    uint32_t* agent_block_d = nullptr;
-    uint32_t* unit_block_d = nullptr;
+    const Instance* inst_d = nullptr;  // tables inside group_tables_
+    const uint32_t* agent_block_d = nullptr;
+    const uint32_t* unit_block_d = nullptr;
   };
   std::vector<Group> groups_;
+  void* group_tables_ = nullptr;        // every group's tables, one allocation
   std::vector<hipStream_t> aux_;        // group streams 1.. (stream 0 is `stream`)
   hipEvent_t fork_ev_ = nullptr;
   std::vector<hipEvent_t> join_ev_;
 
   void free_groups() {
-    for (Group& g : groups_) {
-      dev_free(g.inst_d);
-      dev_free(g.agent_block_d);
-      dev_free(g.unit_block_d);
-    }
+    dev_free(group_tables_);
+    group_tables_ = nullptr;
     groups_.clear();
   }
 
@@ -1355,6 +1353,16 @@ class Engine {
       if (acc * want >= total * start.size()) start.push_back(k + 1);
     }
     start.push_back(count());
+    // each group's rebased instance table and block maps, laid out in one host
+    // blob (256-byte aligned pieces) and uploaded with one copy
+    std::vector<uint8_t> blob;
+    auto put = [&blob](const void* p, size_t bytes) {
+      size_t at = (blob.size() + 255u) & ~size_t(255u);
+      blob.resize(at + bytes);
+      std::memcpy(blob.data() + at, p, bytes);
+      return at;
+    };
+    std::vector<size_t> at_inst, at_agents, at_units;
     for (size_t g = 0; g + 1 < start.size(); g++) {
       Group G;
       G.k0 = start[g];
@@ -1377,16 +1385,20 @@ class Engine {
         G.preload_rows = first || G.preload_rows == r ? r : 0u;
         first = false;
       }
-      groups_.push_back(G);  // owns its tables from here (freed by free_groups)
-      Group& H = groups_.back();
-      SLM_TRY(dev_alloc(&H.inst_d, ins.size() * sizeof(Instance)));
-      SLM_HIP(hipMemcpy(H.inst_d, ins.data(), ins.size() * sizeof(Instance), hipMemcpyHostToDevice));
-      std::vector<uint32_t> am = block_map(ins, H.agents, kAgentBlock, false);
-      SLM_TRY(dev_alloc(&H.agent_block_d, am.size() * 4));
-      SLM_HIP(hipMemcpy(H.agent_block_d, am.data(), am.size() * 4, hipMemcpyHostToDevice));
-      std::vector<uint32_t> um = block_map(ins, H.units, kTrailWavesPerBlock, true);
-      SLM_TRY(dev_alloc(&H.unit_block_d, um.size() * 4));
-      SLM_HIP(hipMemcpy(H.unit_block_d, um.data(), um.size() * 4, hipMemcpyHostToDevice));
+      at_inst.push_back(put(ins.data(), ins.size() * sizeof(Instance)));
+      std::vector<uint32_t> am = block_map(ins, G.agents, kAgentBlock, false);
+      at_agents.push_back(put(am.data(), am.size() * 4));
+      std::vector<uint32_t> um = block_map(ins, G.units, kTrailWavesPerBlock, true);
+      at_units.push_back(put(um.data(), um.size() * 4));
+      groups_.push_back(G);
+    }
+    SLM_TRY(dev_alloc(&group_tables_, blob.size()));
+    SLM_HIP(hipMemcpy(group_tables_, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    const uint8_t* base = static_cast<const uint8_t*>(group_tables_);
+    for (size_t g = 0; g < groups_.size(); g++) {
+      groups_[g].inst_d = reinterpret_cast<const Instance*>(base + at_inst[g]);
+      groups_[g].agent_block_d = reinterpret_cast<const uint32_t*>(base + at_agents[g]);
+      groups_[g].unit_block_d = reinterpret_cast<const uint32_t*>(base + at_units[g]);
     }
     // streams and fork/join events, kept until the engine is released
     const size_t lanes = std::min<size_t>(groups_.size(), kGroupStreams);
