@@ -14,7 +14,9 @@ with the batch's tuning, the plain path on the batched workload
 ("batched_vs_separate").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
-tiles, a re-sort every 20 steps and non-temporal trail-pass loads and stores).
+tiles, a re-sort every 20 steps into 8x8-cell bins and non-temporal trail-pass
+loads and stores; the library's autotuner found it, and a 2000-step sweep
+confirmed it).
 Before its warmup the batched (or combined) leg lets the library's autotuner
 choose its tuning (re-sort interval and bin size, agents per workgroup, trail
 tile rows, tiled sensing, instance groups, streaming) on its own
@@ -49,10 +51,10 @@ WRITE_SIZE_SCALE = 1.0
 
 
 # HBM bytes per launch of the plain leg's dominant kernel at the default
-# workload (4096^2 f32, 2^24 agents, re-sort every 20 steps, 16-row trail
-# tiles, streaming 3): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per
-# dispatch, recorded in profiles/agents_trail_c1_counters.md.
-MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (179790 * 2 + 274482) * 1024}
+# workload (4096^2 f32, 2^24 agents, re-sort every 20 steps into 8x8 bins,
+# 16-row trail tiles, streaming 3): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x
+# 1.0 per dispatch, recorded in profiles/agents_trail_c1_counters.md.
+MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (179778 * 2 + 274500) * 1024}
 
 
 def agent_bytes(sb, pos64=False):
@@ -80,7 +82,7 @@ def parse():
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
     ap.add_argument("--resort", type=int, default=20, help="Morton re-sort interval (0: off)")
     ap.add_argument("--graph-chunk", type=int, default=100)
-    ap.add_argument("--sort-cell-log2", type=int, default=4, help="re-sort bin edge 2^k cells")
+    ap.add_argument("--sort-cell-log2", type=int, default=3, help="re-sort bin edge 2^k cells")
     ap.add_argument("--tiled", action="store_true", help="LDS-staged tiled sensing")
     ap.add_argument("--trail-rows", type=int, default=16,
                     help="plain leg: rows per trail-pass tile (0: chosen from the grid)")
@@ -382,7 +384,8 @@ def main():
     dominant = dict(dominant)
     dominant["traffic"] = MEASURED_TRAFFIC.get(
         (dominant["kernel"], W, H, N, args.storage)
-        if (args.resort, args.trail_rows, args.agents_per_group, args.streaming) == (20, 16, 256, 3)
+        if (args.resort, args.sort_cell_log2, args.trail_rows, args.agents_per_group,
+            args.streaming) == (20, 3, 16, 256, 3)
         else None)
     out = {
         "metric": METRIC,
