@@ -287,7 +287,9 @@ class Engine {
     p.sort_cell_log2 = cfg.sort_cell_log2;
     p.total_tiles = total_tiles_;
     p.tile_inst = tile_inst_d_;
-    p.items = items_d_;
+    // the re-sort builds the tiled pass's work list only when that pass is on
+    // (turning it on re-lays the tables and waits for the next re-sort)
+    p.items = (cfg.flags & SLM_FLAG_TILED_SENSING) ? items_d_ : nullptr;
     p.item_total = item_total_d_;
     p.max_items = max_items_;
     p.resort_interval = cfg.resort_interval;
