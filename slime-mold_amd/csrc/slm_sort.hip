@@ -43,9 +43,10 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
 
 // Bins each agent and takes its rank inside the bin, with as few global
 // atomics as possible: agents re-sorted earlier arrive grouped by bin, so a
-// block's 256 agents usually span a handful of consecutive bins.  Then the
-// lanes of each wave that share a bin are counted with one LDS atomic, and
-// one global atomic per distinct bin of the block reserves the block's range.
+// block's 256 agents usually span a handful of consecutive bins.  Then each
+// agent takes its rank in the block from an LDS atomic on its bin's counter,
+// and one global atomic per distinct bin of the block reserves the block's
+// range.
 // (Dense instances put ~10^4 agents in one bin; a global atomic per wave and
 // bin serialised ~10^2 times more.)  Blocks whose bins span a wider range fall
 // back to one global atomic per wave and bin.  Ranks are not in a fixed order.
@@ -96,21 +97,26 @@ __global__ __launch_bounds__(kAgentBlock) void count_kernel(
     for (uint32_t i = threadIdx.x; i <= kmax - kmin; i += kAgentBlock) s_cnt[i] = 0u;
   __syncthreads();
 
-  uint32_t lane = threadIdx.x & 63u;
-  uint64_t below = (uint64_t(1) << lane) - 1u;
-  uint64_t pending = __ballot(active);
   uint32_t r = 0;
-  while (pending) {
-    int leader = __builtin_ctzll(pending);
-    uint32_t lk = __shfl(k, leader);
-    uint64_t same = __ballot(active && k == lk);
-    uint32_t base = 0;
-    if (static_cast<int>(lane) == leader)
-      base = local ? atomicAdd(&s_cnt[lk - kmin], __popcll(same))
-                   : atomicAdd(count + lk, __popcll(same));
-    base = __shfl(base, leader);
-    if (active && k == lk) r = base + __popcll(same & below);
-    pending &= ~same;
+  if (local) {
+    // rank inside the block: one LDS atomic per agent (lanes of a bin take
+    // distinct old values; the few bins of a block spread over the banks)
+    if (active) r = atomicAdd(&s_cnt[k - kmin], 1u);
+  } else {
+    // wide key range: one global atomic per distinct bin of each wave
+    uint32_t lane = threadIdx.x & 63u;
+    uint64_t below = (uint64_t(1) << lane) - 1u;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+      int leader = __builtin_ctzll(pending);
+      uint32_t lk = __shfl(k, leader);
+      uint64_t same = __ballot(active && k == lk);
+      uint32_t base = 0;
+      if (static_cast<int>(lane) == leader) base = atomicAdd(count + lk, __popcll(same));
+      base = __shfl(base, leader);
+      if (active && k == lk) r = base + __popcll(same & below);
+      pending &= ~same;
+    }
   }
   if (local) {
     __syncthreads();
