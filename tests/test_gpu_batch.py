@@ -188,3 +188,30 @@ def test_instance_groups_switched_mid_run(slm):
         assert slm.lib().slm_batch_set_tuning(b.handle, ctypes.byref(t)) == -5
         b.set_tuning(instance_groups=1, tiled_sensing=1)
         b.step(2)
+
+
+def test_instance_groups_with_params_clear_render_and_mass(slm):
+    """Calls between grouped steps (parameter changes, clear, render, total
+    mass, reads) see every group's work finished, and change nothing."""
+    specs = mixed_specs(slm, seed=6)
+    lut = np.random.default_rng(2).integers(0, 256, 768, dtype=np.uint8).tobytes()
+    outs = []
+    for g in (1, 5):
+        with slm.Batch(specs, storage="bf16", stochastic_rounding=True, graph_chunk=6,
+                       resort_interval=4) as b:
+            b.set_tuning(instance_groups=g, streaming=3)
+            b.step(9)
+            b.set_params(1, preset(slm, "Mesh"))
+            b.step(7)
+            b.clear(3)
+            masses = b.total_mass()
+            images = [b.render(k, lut, 96, 64) for k in range(len(specs))]
+            b.step(13)
+            outs.append((masses, images,
+                         [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]))
+    (m1, i1, s1), (m5, i5, s5) = outs
+    assert np.array_equal(m1, m5)
+    for k in range(len(specs)):
+        assert_bitwise(i5[k], i1[k], f"instance {k} render")
+        assert_bitwise(s5[k][0], s1[k][0], f"instance {k} agents")
+        assert_bitwise(s5[k][1], s1[k][1], f"instance {k} trail")
