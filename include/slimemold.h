@@ -12,10 +12,11 @@
  *   - Every function returns SLM_OK (0) or a negative SLM_ERR_* code and never
  *     aborts or throws.  slm_last_error() gives a message for the calling
  *     thread's most recent failure.
- *   - A simulation handle owns all of its device memory and one HIP stream.
- *     It may be used by one host thread at a time; distinct handles are
- *     independent.  slm_step only enqueues work; the read_* and total_mass
- *     calls wait for it.
+ *   - A simulation handle owns all of its device memory and one HIP stream
+ *     (a batch with instance groups also three helper streams, forked from
+ *     and joined back into its own within every call).  It may be used by
+ *     one host thread at a time; distinct handles are independent.  slm_step
+ *     only enqueues work; the read_* and total_mass calls wait for it.
  *   - Host pointers are borrowed for the duration of the call only.
  *   - Agents are exchanged as the reference lays them out: n records of four
  *     f32 (x, y, angle, speed) (src/compute.wgsl:2,31-32), in the original
