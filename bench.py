@@ -100,6 +100,8 @@ def parse():
     ap.add_argument("--diffuse-steps", type=int, default=500,
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--separate-serial", action="store_true",
+                    help="separate leg: one simulation at a time instead of all streams at once")
     ap.add_argument("--separate-steps", type=int, default=0,
                     help="steps of the batched leg's instances run as separate f32 "
                          "simulations (-1: as many as the batched leg, 0: skip)")
@@ -299,11 +301,13 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
     }
 
 
-def separate_leg(slm, specs, storage, steps, warmup, graph_chunk, tuning):
+def separate_leg(slm, specs, storage, steps, warmup, graph_chunk, tuning, serial=False):
     """The batched leg's instances as separate single simulations (one handle,
     stream and graph per instance), with the batch's tuning: the plain path on
     the same workload.  Each round enqueues one graph chunk per simulation;
-    the streams share the device's hardware queues."""
+    the streams share the device's hardware queues.  serial: each simulation's
+    chunk finishes before the next one's is enqueued (one simulation at a
+    time, as a caller stepping one handle after another)."""
     fields = ("resort_interval", "sort_cell_log2", "agents_per_group", "trail_rows",
               "tiled_sensing")
     log(f"separate: {len(specs)} simulations storage={storage} creating")
@@ -322,6 +326,8 @@ def separate_leg(slm, specs, storage, steps, warmup, graph_chunk, tuning):
                 k = min(chunk, n - done)
                 for sim in sims:
                     sim.step(k)
+                    if serial:
+                        sim.synchronize()
                 done += k
             for sim in sims:
                 sim.synchronize()
@@ -337,6 +343,7 @@ def separate_leg(slm, specs, storage, steps, warmup, graph_chunk, tuning):
         for sim in sims:
             sim.close()
     return {"instances": len(specs), "storage": storage, "steps": steps, "warmup": warmup,
+            "serial": serial,
             "agent_updates_per_s": n_agents * steps / elapsed,
             "cell_updates_per_s": n_cells * steps / elapsed,
             "ms_per_step": elapsed / steps * 1e3,
@@ -451,7 +458,7 @@ def main():
         sep_steps = args.batch_steps if args.separate_steps < 0 else args.separate_steps
         if sep_steps > 0:
             sres = separate_leg(slm, specs, "f32", sep_steps, 20, args.graph_chunk,
-                                bres["tuning"])
+                                bres["tuning"], serial=args.separate_serial)
             out["separate"] = sres
             out["batched_vs_separate"] = round(
                 bres["agent_updates_per_s"] / sres["agent_updates_per_s"], 3)
