@@ -8,10 +8,11 @@ whole grid.
 
 Batched leg: the 256-instance mix (grids 256^2..2048^2, 2^14..2^22 agents,
 parameters swept over the preset range) with a bf16 trail, all instances
-advanced by one agent launch and one trail launch per step.  Separate leg
-(opt-in, --separate-steps): the same instances as 256 separate f32 simulations
-with the batch's tuning, the plain path on the batched workload
-("batched_vs_separate").
+advanced together (autotuned into instance groups on four streams).  Separate
+legs: the same instances as 256 separate f32 simulations with the batch's
+tuning - the plain path on the batched workload - stepped one simulation at a
+time ("batched_vs_separate_serial") and all enqueued at once
+("batched_vs_separate_concurrent").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
 tiles, a re-sort every 20 steps into 8x8-cell bins and non-temporal trail-pass
@@ -100,11 +101,10 @@ def parse():
     ap.add_argument("--diffuse-steps", type=int, default=500,
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
-    ap.add_argument("--separate-serial", action="store_true",
-                    help="separate leg: one simulation at a time instead of all streams at once")
-    ap.add_argument("--separate-steps", type=int, default=0,
+    ap.add_argument("--separate-steps", type=int, default=300,
                     help="steps of the batched leg's instances run as separate f32 "
-                         "simulations (-1: as many as the batched leg, 0: skip)")
+                         "simulations, one at a time and all at once (-1: as many as the "
+                         "batched leg, 0: skip)")
     ap.add_argument("--batch-storage", default="bf16", choices=["f32", "bf16", "f16"])
     ap.add_argument("--rounding", default="stochastic", choices=["nearest", "stochastic"],
                     help="bf16 rounding of the batched, combined and bf16 diffuse legs")
@@ -457,11 +457,13 @@ def main():
         out["batched_vs_plain"] = round(bres["agent_updates_per_s"] / plain["value"], 3)
         sep_steps = args.batch_steps if args.separate_steps < 0 else args.separate_steps
         if sep_steps > 0:
-            sres = separate_leg(slm, specs, "f32", sep_steps, 20, args.graph_chunk,
-                                bres["tuning"], serial=args.separate_serial)
-            out["separate"] = sres
-            out["batched_vs_separate"] = round(
-                bres["agent_updates_per_s"] / sres["agent_updates_per_s"], 3)
+            out["separate"] = {}
+            for mode in ("serial", "concurrent"):
+                sres = separate_leg(slm, specs, "f32", sep_steps, 20, args.graph_chunk,
+                                    bres["tuning"], serial=mode == "serial")
+                out["separate"][mode] = sres
+                out[f"batched_vs_separate_{mode}"] = round(
+                    bres["agent_updates_per_s"] / sres["agent_updates_per_s"], 3)
     if args.cpu_seconds > 0:
         import oracle
 
