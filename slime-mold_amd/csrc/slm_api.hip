@@ -28,17 +28,42 @@ using namespace slm;
 
 namespace {
 
-thread_local std::string g_last_error = "no error";
+// A fixed per-thread buffer, so that recording an error never allocates (and
+// so never throws, even when the error being recorded is a failed allocation).
+thread_local char g_last_error[512] = "no error";
 
 int fail(int code, const char* fmt, ...) {
-  char buf[512];
+  char buf[sizeof g_last_error];
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  g_last_error = buf;
+  memcpy(g_last_error, buf, sizeof buf);
   return code;
 }
+
+// Converts the exception in flight into a status code.  Host-side containers
+// (staging copies of a trail, graph tables, group tables) can throw
+// std::bad_alloc; nothing may unwind through the C ABI, so every entry point
+// below wraps its body in SLM_ENTRY_BEGIN / SLM_ENTRY_END.
+int fail_exception() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return fail(SLM_ERR_OUT_OF_MEMORY, "host memory allocation failed");
+  } catch (const std::exception& e) {
+    return fail(SLM_ERR_HIP, "internal error: %s", e.what());
+  } catch (...) {
+    return fail(SLM_ERR_HIP, "internal error: unknown exception");
+  }
+}
+
+#define SLM_ENTRY_BEGIN try {
+#define SLM_ENTRY_END      \
+  }                        \
+  catch (...) {            \
+    return fail_exception(); \
+  }
 
 #define SLM_HIP(call)                                                                 \
   do {                                                                                \
@@ -1757,9 +1782,10 @@ struct slm_batch {
 
 extern "C" {
 
-const char* slm_last_error(void) { return g_last_error.c_str(); }
+const char* slm_last_error(void) { return g_last_error; }
 
 int slm_device_count(int* out_count) {
+  SLM_ENTRY_BEGIN
   if (!out_count) return fail(SLM_ERR_INVALID_ARGUMENT, "out_count is NULL");
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -1770,9 +1796,11 @@ int slm_device_count(int* out_count) {
   }
   *out_count = n;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_config_default(slm_config* cfg) {
+  SLM_ENTRY_BEGIN
   if (!cfg) return fail(SLM_ERR_INVALID_ARGUMENT, "cfg is NULL");
   std::memset(cfg, 0, sizeof *cfg);
   cfg->width = 1600;  // src/settings.rs:4-5
@@ -1781,9 +1809,11 @@ int slm_config_default(slm_config* cfg) {
   cfg->graph_chunk = 100;
   cfg->sort_cell_log2 = 4;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_params_default(slm_params* p) {
+  SLM_ENTRY_BEGIN
   if (!p) return fail(SLM_ERR_INVALID_ARGUMENT, "params is NULL");
   p->decay_factor = 10.0f;  // src/settings.rs:9-27
   p->agent_jitter = 0.0f;
@@ -1797,11 +1827,13 @@ int slm_params_default(slm_params* p) {
   p->blur_radius = 2.0f;
   p->blur_sigma = 1.0f;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 // ---- single simulation ------------------------------------------------------
 
 int slm_create(const slm_config* cfg, const slm_params* params, slm_sim** out) {
+  SLM_ENTRY_BEGIN
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
   if (!cfg) return fail(SLM_ERR_INVALID_ARGUMENT, "cfg is NULL");
@@ -1827,6 +1859,7 @@ int slm_create(const slm_config* cfg, const slm_params* params, slm_sim** out) {
   }
   *out = s;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 #define SIM_GUARD(s)                                                      \
@@ -1834,67 +1867,87 @@ int slm_create(const slm_config* cfg, const slm_params* params, slm_sim** out) {
   DeviceGuard guard_((s)->eng->cfg.device)
 
 int slm_destroy(slm_sim* s) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   delete s->eng;
   delete s;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_set_params(slm_sim* s, const slm_params* p) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   SLM_TRY(check_params(p));
   return s->eng->set_params(0, *p);
+  SLM_ENTRY_END
 }
 
 int slm_get_params(const slm_sim* s, slm_params* out) {
+  SLM_ENTRY_BEGIN
   if (!s || !s->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = s->eng->spec[0].params;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_step(slm_sim* s, uint32_t n) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   return s->eng->step(n);
+  SLM_ENTRY_END
 }
 
 int slm_step_timed(slm_sim* s, uint32_t n, slm_timing* out) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   return s->eng->step_timed(n, out);
+  SLM_ENTRY_END
 }
 
 int slm_synchronize(slm_sim* s) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   return s->eng->sync();
+  SLM_ENTRY_END
 }
 
 int slm_clear(slm_sim* s) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   return s->eng->clear(0);
+  SLM_ENTRY_END
 }
 
 int slm_resize(slm_sim* s, uint32_t w, uint32_t h) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   SLM_TRY(check_grid(w, h));
   SLM_TRY(s->eng->resize(w, h));
   s->cfg.width = w;
   s->cfg.height = h;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_set_agent_count(slm_sim* s, uint64_t n) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (n > kMaxAgents) return fail(SLM_ERR_INVALID_ARGUMENT, "agent_count must be < 2^32 - 256");
   SLM_TRY(s->eng->set_agent_count(n));
   s->cfg.agent_count = n;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_reassign_speeds(slm_sim* s, float smin, float smax) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!std::isfinite(smin) || !std::isfinite(smax))
     return fail(SLM_ERR_INVALID_ARGUMENT, "speeds must be finite");
   return s->eng->reassign_speeds(0, smin, smax);
+  SLM_ENTRY_END
 }
 
 static int check_n(uint64_t n, uint64_t want, const char* what) {
@@ -1905,40 +1958,51 @@ static int check_n(uint64_t n, uint64_t want, const char* what) {
 }
 
 int slm_read_agents(slm_sim* s, float* out, uint64_t n) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(n, s->eng->spec[0].agents, "read_agents"));
   return s->eng->read_agents(0, out);
+  SLM_ENTRY_END
 }
 
 int slm_write_agents(slm_sim* s, const float* in, uint64_t n) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(n, s->eng->spec[0].agents, "write_agents"));
   return s->eng->write_agents(0, in);
+  SLM_ENTRY_END
 }
 
 int slm_read_trail(slm_sim* s, float* out, uint64_t cells) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(cells, s->eng->cells(0), "read_trail (cells != width*height)"));
   return s->eng->read_trail(0, out);
+  SLM_ENTRY_END
 }
 
 int slm_write_trail(slm_sim* s, const float* in, uint64_t cells) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(cells, s->eng->cells(0), "write_trail (cells != width*height)"));
   return s->eng->write_trail(0, in);
+  SLM_ENTRY_END
 }
 
 int slm_total_mass(slm_sim* s, double* out) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   return s->eng->total_mass(out);
+  SLM_ENTRY_END
 }
 
 int slm_get_info(const slm_sim* s, slm_info* out) {
+  SLM_ENTRY_BEGIN
   if (!s || !s->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   std::memset(out, 0, sizeof *out);
   const Engine& e = *s->eng;
@@ -1952,12 +2016,15 @@ int slm_get_info(const slm_sim* s, slm_info* out) {
   out->resort_interval = e.cfg.resort_interval;
   out->graph_chunk = e.cfg.graph_chunk;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_get_tuning(const slm_sim* s, slm_tuning* out) {
+  SLM_ENTRY_BEGIN
   if (!s || !s->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = s->eng->tuning();
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 static void sync_config(slm_config& c, const Engine& e) {
@@ -1967,19 +2034,23 @@ static void sync_config(slm_config& c, const Engine& e) {
 }
 
 int slm_set_tuning(slm_sim* s, const slm_tuning* t) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!t) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_tuning(*t));
   int rc = s->eng->set_tuning(*t);
   sync_config(s->cfg, *s->eng);
   return rc;
+  SLM_ENTRY_END
 }
 
 int slm_autotune(slm_sim* s, uint32_t trial_steps, slm_tuning* out) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   int rc = s->eng->autotune(trial_steps, out);
   sync_config(s->cfg, *s->eng);
   return rc;
+  SLM_ENTRY_END
 }
 
 static int check_render(const uint8_t* lut, uint32_t tex_w, uint32_t tex_h, const void* out) {
@@ -1992,32 +2063,41 @@ static int check_render(const uint8_t* lut, uint32_t tex_w, uint32_t tex_h, cons
 
 int slm_render_rgba8(slm_sim* s, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
                      uint8_t* rgba) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   SLM_TRY(check_render(lut, tex_w, tex_h, rgba));
   return s->eng->render(0, lut, tex_w, tex_h, rgba, false);
+  SLM_ENTRY_END
 }
 
 int slm_render_rgba8_device(slm_sim* s, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
                             void* rgba_device) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   SLM_TRY(check_render(lut, tex_w, tex_h, rgba_device));
   return s->eng->render(0, lut, tex_w, tex_h, rgba_device, true);
+  SLM_ENTRY_END
 }
 
 int slm_snapshot_size(const slm_sim* s, uint64_t* bytes) {
+  SLM_ENTRY_BEGIN
   if (!s || !s->eng || !bytes) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   *bytes = s->eng->snapshot_bytes();
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_save_snapshot(slm_sim* s, void* buf, uint64_t bytes) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   if (!buf) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(bytes, s->eng->snapshot_bytes(), "save_snapshot (bytes != slm_snapshot_size)"));
   return s->eng->save_snapshot(static_cast<uint8_t*>(buf));
+  SLM_ENTRY_END
 }
 
 int slm_load_snapshot(slm_sim* s, const void* buf, uint64_t bytes) {
+  SLM_ENTRY_BEGIN
   SIM_GUARD(s);
   SnapHeader hd;
   SLM_TRY(check_snapshot(static_cast<const uint8_t*>(buf), bytes, &hd));
@@ -2028,9 +2108,11 @@ int slm_load_snapshot(slm_sim* s, const void* buf, uint64_t bytes) {
     return fail(SLM_ERR_INVALID_ARGUMENT,
                 "snapshot shape (grid, agents, storage, positions) differs from the simulation's");
   return s->eng->load_snapshot(static_cast<const uint8_t*>(buf));
+  SLM_ENTRY_END
 }
 
 int slm_create_from_snapshot(const void* buf, uint64_t bytes, int32_t device, slm_sim** out) {
+  SLM_ENTRY_BEGIN
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
   SnapHeader hd;
@@ -2048,27 +2130,32 @@ int slm_create_from_snapshot(const void* buf, uint64_t bytes, int32_t device, sl
   SLM_TRY(slm_create(&cfg, &hd.params, &s));
   int rc = slm_load_snapshot(s, buf, bytes);
   if (rc) {
-    std::string msg = g_last_error;
+    char msg[sizeof g_last_error];
+    memcpy(msg, g_last_error, sizeof msg);
     slm_destroy(s);
-    return fail(rc, "%s", msg.c_str());
+    return fail(rc, "%s", msg);
   }
   *out = s;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 // ---- batches ------------------------------------------------------------------
 
 int slm_batch_config_default(slm_batch_config* cfg) {
+  SLM_ENTRY_BEGIN
   if (!cfg) return fail(SLM_ERR_INVALID_ARGUMENT, "cfg is NULL");
   std::memset(cfg, 0, sizeof *cfg);
   cfg->storage = SLM_STORAGE_F32;
   cfg->graph_chunk = 100;
   cfg->sort_cell_log2 = 4;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts, uint32_t count,
                      slm_batch** out) {
+  SLM_ENTRY_BEGIN
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
   if (!cfg || !insts) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -2089,7 +2176,7 @@ int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts
     const slm_instance_desc& d = insts[k];
     int rc = check_grid(d.width, d.height);
     if (!rc) rc = check_params(&d.params);
-    if (rc) return fail(rc, "instance %u: %s", k, g_last_error.c_str());
+    if (rc) return fail(rc, "instance %u: %s", k, g_last_error);
     agents += d.agent_count;
     cells += static_cast<uint64_t>(d.width) * d.height;
     specs.push_back({d.width, d.height, d.agent_count, d.seed, d.params});
@@ -2105,6 +2192,7 @@ int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts
   }
   *out = b;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 #define BATCH_GUARD(b)                                                        \
@@ -2117,78 +2205,100 @@ int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts
               (b)->eng->count())
 
 int slm_batch_destroy(slm_batch* b) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   delete b->eng;
   delete b;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_batch_step(slm_batch* b, uint32_t n) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   return b->eng->step(n);
+  SLM_ENTRY_END
 }
 
 int slm_batch_step_timed(slm_batch* b, uint32_t n, slm_timing* out) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   return b->eng->step_timed(n, out);
+  SLM_ENTRY_END
 }
 
 int slm_batch_synchronize(slm_batch* b) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   return b->eng->sync();
+  SLM_ENTRY_END
 }
 
 int slm_batch_set_params(slm_batch* b, uint32_t k, const slm_params* p) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
   SLM_TRY(check_params(p));
   return b->eng->set_params(k, *p);
+  SLM_ENTRY_END
 }
 
 int slm_batch_clear(slm_batch* b, uint32_t k) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
   return b->eng->clear(k);
+  SLM_ENTRY_END
 }
 
 int slm_batch_read_agents(slm_batch* b, uint32_t k, float* out, uint64_t n) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(n, b->eng->spec[k].agents, "read_agents"));
   return b->eng->read_agents(k, out);
+  SLM_ENTRY_END
 }
 
 int slm_batch_write_agents(slm_batch* b, uint32_t k, const float* in, uint64_t n) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
   if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(n, b->eng->spec[k].agents, "write_agents"));
   return b->eng->write_agents(k, in);
+  SLM_ENTRY_END
 }
 
 int slm_batch_read_trail(slm_batch* b, uint32_t k, float* out, uint64_t cells) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(cells, b->eng->cells(k), "read_trail (cells != width*height)"));
   return b->eng->read_trail(k, out);
+  SLM_ENTRY_END
 }
 
 int slm_batch_write_trail(slm_batch* b, uint32_t k, const float* in, uint64_t cells) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
   if (!in) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(cells, b->eng->cells(k), "write_trail (cells != width*height)"));
   return b->eng->write_trail(k, in);
+  SLM_ENTRY_END
 }
 
 int slm_batch_total_mass(slm_batch* b, double* out, uint32_t count) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_n(count, b->eng->count(), "total_mass (count != instances)"));
   return b->eng->total_mass(out);
+  SLM_ENTRY_END
 }
 
 static void sync_batch_config(slm_batch_config& c, const Engine& e) {
@@ -2198,36 +2308,45 @@ static void sync_batch_config(slm_batch_config& c, const Engine& e) {
 }
 
 int slm_batch_get_tuning(const slm_batch* b, slm_tuning* out) {
+  SLM_ENTRY_BEGIN
   if (!b || !b->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = b->eng->tuning();
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 int slm_batch_set_tuning(slm_batch* b, const slm_tuning* t) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   if (!t) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   SLM_TRY(check_tuning(*t));
   int rc = b->eng->set_tuning(*t);
   sync_batch_config(b->cfg, *b->eng);
   return rc;
+  SLM_ENTRY_END
 }
 
 int slm_batch_autotune(slm_batch* b, uint32_t trial_steps, slm_tuning* out) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   int rc = b->eng->autotune(trial_steps, out);
   sync_batch_config(b->cfg, *b->eng);
   return rc;
+  SLM_ENTRY_END
 }
 
 int slm_batch_render_rgba8(slm_batch* b, uint32_t k, const uint8_t* lut, uint32_t tex_w,
                            uint32_t tex_h, uint8_t* rgba) {
+  SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
   SLM_TRY(check_render(lut, tex_w, tex_h, rgba));
   return b->eng->render(k, lut, tex_w, tex_h, rgba, false);
+  SLM_ENTRY_END
 }
 
 int slm_batch_get_info(const slm_batch* b, slm_batch_info* out) {
+  SLM_ENTRY_BEGIN
   if (!b || !b->eng || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
   std::memset(out, 0, sizeof *out);
   const Engine& e = *b->eng;
@@ -2240,6 +2359,7 @@ int slm_batch_get_info(const slm_batch* b, slm_batch_info* out) {
   out->steps_done = e.steps_done;
   out->device_bytes = e.device_bytes;
   return SLM_OK;
+  SLM_ENTRY_END
 }
 
 }  // extern "C"
