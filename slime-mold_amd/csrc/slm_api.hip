@@ -1117,10 +1117,7 @@ class Engine {
       SLM_TRY(dev_alloc(&a.v, n * 4));
       if (sorting()) SLM_TRY(dev_alloc(&a.id, n * 4));
     }
-    if (sorting()) {
-      SLM_TRY(dev_alloc(&sort_.key, n * 4));
-      SLM_TRY(dev_alloc(&sort_.rank, n * 4));
-    }
+    if (sorting()) SLM_TRY(dev_alloc(&sort_.rank, n * 4));
     acur_ = 0;
     // block -> first instance whose slots reach into the block
     uint32_t nb = blocks_for(n, kAgentBlock);
@@ -1149,9 +1146,8 @@ class Engine {
       dev_free(a.id);
       a = AgentSoA{};
     }
-    dev_free(sort_.key);
     dev_free(sort_.rank);
-    sort_.key = sort_.rank = nullptr;
+    sort_.rank = nullptr;
     dev_free(agent_block_d_);
     agent_block_d_ = nullptr;
   }

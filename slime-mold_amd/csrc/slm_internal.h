@@ -168,8 +168,7 @@ void launch_unpack_positions64(const Instance& in, const AgentSoA& a, const doub
 
 // ---- Morton re-sort ------------------------------------------------------
 struct SortScratch {
-  uint32_t* key;       // per slot: global bin
-  uint32_t* rank;      // per slot: rank inside its bin
+  uint32_t* rank;      // per slot: rank inside its bin (the bin is recomputed from x, y)
   uint32_t* count;     // per bin
   uint32_t* offset;    // per bin, exclusive prefix sum; offset[total_bins] = total agents
   uint32_t* tile_items;  // per tile: work items, then their exclusive prefix sum
