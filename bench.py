@@ -15,9 +15,9 @@ time ("batched_vs_separate_serial") and all enqueued at once
 ("batched_vs_separate_concurrent").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
-tiles, a re-sort every 20 steps into 8x8-cell bins and non-temporal trail-pass
-loads and stores; the library's autotuner found it, and a 2000-step sweep
-confirmed it).
+tiles, a re-sort every 15 steps into 8x8-cell bins and non-temporal trail-pass
+loads and stores; the library's autotuner found the bins, and 2000-step sweeps
+chose the interval and the streaming mode).
 Before its warmup the batched (or combined) leg lets the library's autotuner
 choose its tuning (re-sort interval and bin size, agents per workgroup, trail
 tile rows, tiled sensing, instance groups, streaming) on its own
@@ -52,10 +52,11 @@ WRITE_SIZE_SCALE = 1.0
 
 
 # HBM bytes per launch of the plain leg's dominant kernel at the default
-# workload (4096^2 f32, 2^24 agents, re-sort every 20 steps into 8x8 bins,
+# workload (4096^2 f32, 2^24 agents, re-sort every 15 steps into 8x8 bins,
 # 16-row trail tiles, streaming 3): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x
-# 1.0 per dispatch, recorded in profiles/agents_trail_c1_counters.md.
-MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (179778 * 2 + 274500) * 1024}
+# 1.0 per dispatch averaged over the 2000 timed steps, recorded in
+# profiles/agents_trail_c1_counters.md.
+MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (168985 * 2 + 218185) * 1024}
 
 
 def agent_bytes(sb, pos64=False):
@@ -81,7 +82,10 @@ def parse():
     ap.add_argument("--agents-log2", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
-    ap.add_argument("--resort", type=int, default=20, help="Morton re-sort interval (0: off)")
+    ap.add_argument("--resort", type=int, default=15,
+                    help="plain leg's Morton re-sort interval (0: off)")
+    ap.add_argument("--batch-resort", type=int, default=20,
+                    help="batched/combined legs' re-sort interval before autotuning")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--sort-cell-log2", type=int, default=3, help="re-sort bin edge 2^k cells")
     ap.add_argument("--tiled", action="store_true", help="LDS-staged tiled sensing")
@@ -370,7 +374,7 @@ def main():
 
     if args.workload == "combined":
         specs = slm.workloads.combined()
-        res = batch_leg(slm, specs, "bf16", args.steps, args.warmup, args.resort,
+        res = batch_leg(slm, specs, "bf16", args.steps, args.warmup, args.batch_resort,
                         args.graph_chunk, args.sort_cell_log2, "combined", args.tiled,
                         args.autotune, args.rounding)
         out = {
@@ -392,7 +396,7 @@ def main():
     dominant["traffic"] = MEASURED_TRAFFIC.get(
         (dominant["kernel"], W, H, N, args.storage)
         if (args.resort, args.sort_cell_log2, args.trail_rows, args.agents_per_group,
-            args.streaming) == (20, 3, 16, 256, 3)
+            args.streaming) == (15, 3, 16, 256, 3)
         else None)
     out = {
         "metric": METRIC,
@@ -450,7 +454,7 @@ def main():
                                           args.graph_chunk, args.autotune, args.rounding)]
     if args.batch_steps > 0:
         specs = slm.workloads.batch_mix(args.batch_count)
-        bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.resort,
+        bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.batch_resort,
                          args.graph_chunk, args.sort_cell_log2, "batched", args.tiled,
                          args.autotune, args.rounding)
         out["batched"] = bres
