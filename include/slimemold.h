@@ -12,6 +12,12 @@
  *   - Every function returns SLM_OK (0) or a negative SLM_ERR_* code and never
  *     aborts or throws.  slm_last_error() gives a message for the calling
  *     thread's most recent failure.
+ *   - A failed call leaves the handle usable, with one exception: when
+ *     slm_resize, slm_set_agent_count or a tuning change (slm_set_tuning,
+ *     slm_autotune and their batch forms) fails part-way through
+ *     re-laying its buffers, every later call on that handle but destroy
+ *     returns the same status.  (Inside slm_autotune, a trial that fails
+ *     this way is undone when the final re-layout and restore succeed.)
  *   - A simulation handle owns all of its device memory and one HIP stream
  *     (a batch with instance groups also three helper streams, forked from
  *     and joined back into its own within every call).  It may be used by

@@ -275,6 +275,11 @@ class Engine {
   hipStream_t stream = nullptr;
   uint64_t steps_done = 0;
   uint64_t device_bytes = 0;
+  // A failure part-way through a re-layout (resize, agent count, tuning) can
+  // leave buffers freed or tables out of step with them.  The status is kept
+  // here and the handle then refuses every call but destroy, rather than
+  // launch kernels on them.
+  int broken = SLM_OK;
 
   ~Engine() { release(); }
 
@@ -491,26 +496,30 @@ class Engine {
   // ---- single-simulation reshaping ----------------------------------------
   int resize(uint32_t w, uint32_t h) {  // pool of one
     SLM_TRY(sync());
-    drop_graphs();
-    float sx = static_cast<float>(w) / static_cast<float>(spec[0].width);
-    float sy = static_cast<float>(h) / static_cast<float>(spec[0].height);
-    launch_scale_positions(inst[0], agents_[acur_], pos64() ? 1 : 0, sx, sy, stream);
-    SLM_TRY(sync());
-    spec[0].width = w;
-    spec[0].height = h;
-    free_trail_layout();
-    return layout_trail();
+    return relayout([&]() -> int {
+      drop_graphs();
+      float sx = static_cast<float>(w) / static_cast<float>(spec[0].width);
+      float sy = static_cast<float>(h) / static_cast<float>(spec[0].height);
+      launch_scale_positions(inst[0], agents_[acur_], pos64() ? 1 : 0, sx, sy, stream);
+      SLM_TRY(sync());
+      spec[0].width = w;
+      spec[0].height = h;
+      free_trail_layout();
+      return layout_trail();
+    }());
   }
 
   int set_agent_count(uint64_t n) {  // pool of one
     SLM_TRY(sync());
-    drop_graphs();
-    free_agent_layout();
-    spec[0].agents = n;
-    SLM_TRY(layout_agents());
-    SLM_TRY(layout_trail_tables());
-    init_agents(0);
-    return sync_checked("agent re-initialisation");
+    return relayout([&]() -> int {
+      drop_graphs();
+      free_agent_layout();
+      spec[0].agents = n;
+      SLM_TRY(layout_agents());
+      SLM_TRY(layout_trail_tables());
+      init_agents(0);
+      return sync_checked("agent re-initialisation");
+    }());
   }
 
   // ---- display colour map -------------------------------------------------------
@@ -671,8 +680,8 @@ class Engine {
     Snapshot snap;
     int rc = take_snapshot(snap, false);
     if (rc == SLM_OK) rc = apply_tuning(t);
-    if (rc == SLM_OK) rc = restore_snapshot(snap);
-    if (rc == SLM_OK) rc = sync_checked("re-tuning");
+    if (rc == SLM_OK) rc = relayout(restore_snapshot(snap));
+    if (rc == SLM_OK) rc = relayout(sync_checked("re-tuning"));
     free_snapshot(snap);
     return rc;
   }
@@ -694,6 +703,9 @@ class Engine {
     trials_ = 0;
     rc = search(snap, trial_steps ? trial_steps : 1u, best, bt);
     int rc2 = put_tuning(rc == SLM_OK ? best : start, snap);
+    // a tuning applied and the state restored in full: consistent again, even
+    // if a trial failed part-way
+    if (rc2 == SLM_OK) broken = SLM_OK;
     steps_done = steps0;
     free_snapshot(snap);
     if (rc) return rc;
@@ -783,7 +795,7 @@ class Engine {
     if (sort != sorting()) {
       uint64_t n = total_agents_;
       uint64_t pos = pos64() ? 16 : 8;
-      need += n * ((pos + 8 + (sort ? 4 : 0)) * (sort ? 2 : 1) + (sort ? 8 : 0));  // new arrays
+      need += n * ((pos + 8 + (sort ? 4 : 0)) * (sort ? 2 : 1) + (sort ? 4 : 0));  // new arrays
       need += n * (pos + 8);                                                      // snapshot
     }
     size_t free_b = 0, total_b = 0;
@@ -798,27 +810,34 @@ class Engine {
   // re-sorting on or off reallocates the agent arrays, losing their contents:
   // the caller restores them.
   int apply_tuning(const slm_tuning& t) {
-    bool relayout = sorting() != (t.resort_interval > 0);
-    drop_graphs();
-    if (relayout) free_agent_layout();
-    cfg.resort_interval = t.resort_interval;
-    cfg.sort_cell_log2 = t.sort_cell_log2;
-    cfg.agents_per_group = t.agents_per_group;
-    cfg.trail_rows = t.trail_rows;
-    cfg.instance_groups = t.instance_groups ? t.instance_groups : 1u;
-    cfg.streaming = t.streaming;
-    if (t.tiled_sensing)
-      cfg.flags |= SLM_FLAG_TILED_SENSING;
-    else
-      cfg.flags &= ~SLM_FLAG_TILED_SENSING;
-    if (relayout) SLM_TRY(layout_agents());
-    return layout_trail_tables();
+    return relayout([&]() -> int {
+      bool new_agents = sorting() != (t.resort_interval > 0);
+      drop_graphs();
+      if (new_agents) free_agent_layout();
+      cfg.resort_interval = t.resort_interval;
+      cfg.sort_cell_log2 = t.sort_cell_log2;
+      cfg.agents_per_group = t.agents_per_group;
+      cfg.trail_rows = t.trail_rows;
+      cfg.instance_groups = t.instance_groups ? t.instance_groups : 1u;
+      cfg.streaming = t.streaming;
+      if (t.tiled_sensing)
+        cfg.flags |= SLM_FLAG_TILED_SENSING;
+      else
+        cfg.flags &= ~SLM_FLAG_TILED_SENSING;
+      if (new_agents) SLM_TRY(layout_agents());
+      return layout_trail_tables();
+    }());
+  }
+
+  int relayout(int rc) {
+    if (rc != SLM_OK && broken == SLM_OK) broken = rc;
+    return rc;
   }
 
   int put_tuning(const slm_tuning& t, const Snapshot& snap) {
     SLM_TRY(apply_tuning(t));
-    SLM_TRY(restore_snapshot(snap));
-    return sync_checked("autotune restore");
+    SLM_TRY(relayout(restore_snapshot(snap)));
+    return relayout(sync_checked("autotune restore"));
   }
 
   // Times `t` over at least `steps` steps (whole re-sort intervals) from the
@@ -1858,13 +1877,19 @@ int slm_create(const slm_config* cfg, const slm_params* params, slm_sim** out) {
   SLM_ENTRY_END
 }
 
-#define SIM_GUARD(s)                                                      \
+#define SIM_HANDLE(s)                                                      \
   if (!(s) || !(s)->eng) return fail(SLM_ERR_INVALID_ARGUMENT, "sim is NULL"); \
   DeviceGuard guard_((s)->eng->cfg.device)
 
+#define SIM_GUARD(s)                                                                  \
+  SIM_HANDLE(s);                                                                      \
+  if ((s)->eng->broken)                                                               \
+  return fail((s)->eng->broken, "the simulation is unusable after a failed resize, agent " \
+              "count or tuning change; destroy it")
+
 int slm_destroy(slm_sim* s) {
   SLM_ENTRY_BEGIN
-  SIM_GUARD(s);
+  SIM_HANDLE(s);
   delete s->eng;
   delete s;
   return SLM_OK;
@@ -2191,9 +2216,14 @@ int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts
   SLM_ENTRY_END
 }
 
-#define BATCH_GUARD(b)                                                        \
+#define BATCH_HANDLE(b)                                                        \
   if (!(b) || !(b)->eng) return fail(SLM_ERR_INVALID_ARGUMENT, "batch is NULL"); \
   DeviceGuard guard_((b)->eng->cfg.device)
+
+#define BATCH_GUARD(b)                                                                     \
+  BATCH_HANDLE(b);                                                                         \
+  if ((b)->eng->broken)                                                                    \
+  return fail((b)->eng->broken, "the batch is unusable after a failed tuning change; destroy it")
 
 #define BATCH_INDEX(b, k)                                                                \
   if ((k) >= (b)->eng->count())                                                          \
@@ -2202,7 +2232,7 @@ int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts
 
 int slm_batch_destroy(slm_batch* b) {
   SLM_ENTRY_BEGIN
-  BATCH_GUARD(b);
+  BATCH_HANDLE(b);
   delete b->eng;
   delete b;
   return SLM_OK;
