@@ -1173,8 +1173,7 @@ class Engine {
 
   // Trail buffers, trail-pass units, re-sort bins and their tables.
   int layout_trail() {
-    uint64_t off = 0, total_cells = 0;
-    for (size_t k = 0; k < spec.size(); k++) total_cells += cells(k);
+    uint64_t off = 0;
     for (size_t k = 0; k < spec.size(); k++) {
       Instance& in = inst[k];
       in.width = spec[k].width;
@@ -1188,7 +1187,6 @@ class Engine {
       SLM_HIP(hipMemsetAsync(trail_[b], 0, trail_elems_ * storage_bytes(cfg.storage), stream));
     }
     cur_ = 0;
-    (void)total_cells;
     return layout_trail_tables();
   }
 
