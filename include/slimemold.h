@@ -93,7 +93,12 @@ typedef struct slm_config {
   uint64_t seed;            /* agent initialisation seed (SplitMix64) */
   int32_t storage;          /* slm_storage */
   int32_t device;           /* HIP device ordinal */
-  uint32_t resort_interval; /* Morton re-sort of agents every k steps; 0 = never */
+  uint32_t resort_interval; /* Morton re-sort of agents every k steps; 0 = never.
+                               Inside a captured chunk the re-sorts fall on the
+                               chunk's steps 0, k, 2k, ..., so a chunk length
+                               that k does not divide re-sorts a little more
+                               often.  Results do not depend on when re-sorts
+                               happen. */
   uint32_t graph_chunk;     /* steps per captured hipGraph; 0 = launch eagerly */
   uint32_t sort_cell_log2;  /* re-sort bin edge = 2^k cells (default 4) */
   uint32_t flags;           /* SLM_FLAG_* */
