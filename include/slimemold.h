@@ -125,6 +125,16 @@ typedef struct slm_config {
  * long round-to-nearest BF16 run's mass.  Other storage: SLM_ERR_UNSUPPORTED.
  * Values given to slm_write_trail are still rounded to nearest. */
 #define SLM_FLAG_STOCHASTIC_ROUNDING 0x8u
+/* Diffuse with a separable Gaussian instead of the reference's 3x3 box
+ * (an extension: the reference carries blur_radius and blur_sigma but never
+ * reads them).  Per instance, R = round(blur_radius) must be 1..8 and
+ * blur_sigma > 0, else SLM_ERR_INVALID_ARGUMENT.  With T2 the deposited and
+ * decayed trail and w[i] = exp(-i^2 / (2 sigma^2)) / sum_{|j|<=R} exp(...)
+ * (double, rounded to f32), the pass computes in f32, no fused operations:
+ *   H[y][x] = sum_{i=-R..R} w[|i|] * T2[y][(x+i) mod W]   (i ascending)
+ *   G[y][x] = sum_{j=-R..R} w[|j|] * H[(y+j) mod H][x]    (j ascending)
+ *   out     = mix(T2[y][x], G[y][x], clamp(diffusion_rate, 0, 1)). */
+#define SLM_FLAG_GAUSSIAN_DIFFUSE 0x10u
 
 typedef struct slm_info {
   uint32_t width, height;

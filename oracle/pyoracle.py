@@ -69,6 +69,11 @@ def lib():
                                             ctypes.c_uint32, ctypes.c_int]),
             "orc_trail_pass_f64": (None, [_D, _U8, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.POINTER(OrcParams), _D, ctypes.c_int]),
+            "orc_trail_pass_gauss": (None, [_F, _U8, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.POINTER(OrcParams), ctypes.c_int, _F,
+                                            ctypes.c_int]),
+            "orc_gauss_weights": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, _F]),
+            "orc_set_gaussian": (None, [ctypes.c_int]),
             "orc_total_mass": (ctypes.c_double, [_F, ctypes.c_uint64]),
             "orc_total_mass_f64": (ctypes.c_double, [_D, ctypes.c_uint64]),
             "orc_render": (None, [_F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
@@ -135,7 +140,8 @@ def agent_update(trail: np.ndarray, p, index: int, agent) -> tuple:
     return a, d
 
 
-def trail_pass(trail: np.ndarray, p, mark=None, storage: str = "f32", nthreads: int = 0) -> np.ndarray:
+def trail_pass(trail: np.ndarray, p, mark=None, storage: str = "f32", nthreads: int = 0,
+               gaussian: bool = False) -> np.ndarray:
     t = np.ascontiguousarray(trail, dtype=np.float32)
     out = np.empty_like(t)
     op = params(p)
@@ -143,9 +149,17 @@ def trail_pass(trail: np.ndarray, p, mark=None, storage: str = "f32", nthreads: 
     if mark is not None:
         mk = np.ascontiguousarray(mark, dtype=np.uint8)
         m = mk.ctypes.data_as(_U8)
-    lib().orc_trail_pass(_f(t), m, t.shape[1], t.shape[0], ctypes.byref(op), STORAGE[storage],
-                         _f(out), nthreads)
+    fn = lib().orc_trail_pass_gauss if gaussian else lib().orc_trail_pass
+    fn(_f(t), m, t.shape[1], t.shape[0], ctypes.byref(op), STORAGE[storage], _f(out), nthreads)
     return out
+
+
+def gauss_weights(radius: float, sigma: float):
+    """(R, float32 weights w[0..R]) of the Gaussian diffuse extension; R = 0 if invalid."""
+    w = np.zeros(9, np.float32)
+    r = lib().orc_gauss_weights(radius, sigma, _f(w))
+    return r, w[:r + 1].copy()
+
 
 
 def trail_pass_f64(trail: np.ndarray, p, mark=None, nthreads: int = 0) -> np.ndarray:
@@ -172,26 +186,34 @@ def deposit_marks(agents: np.ndarray, w: int, h: int) -> np.ndarray:
 
 
 def step(agents: np.ndarray, trail: np.ndarray, p, n_steps: int = 1, storage: str = "f32",
-         nthreads: int = 0) -> None:
+         nthreads: int = 0, gaussian: bool = False) -> None:
     """n_steps in place on (n,4) float32 agents and (h,w) float32 trail."""
     assert agents.dtype == np.float32 and agents.flags.c_contiguous
     assert trail.dtype == np.float32 and trail.flags.c_contiguous
     op = params(p)
-    rc = lib().orc_step(trail.shape[1], trail.shape[0], agents.shape[0], _f(agents), _f(trail),
-                        ctypes.byref(op), STORAGE[storage], n_steps, nthreads)
+    lib().orc_set_gaussian(1 if gaussian else 0)
+    try:
+        rc = lib().orc_step(trail.shape[1], trail.shape[0], agents.shape[0], _f(agents),
+                            _f(trail), ctypes.byref(op), STORAGE[storage], n_steps, nthreads)
+    finally:
+        lib().orc_set_gaussian(0)
     if rc != 0:
         raise RuntimeError(f"orc_step failed: {rc}")
 
 
 def step_p64(agents: np.ndarray, trail: np.ndarray, p, n_steps: int = 1, storage: str = "f32",
-             nthreads: int = 0) -> None:
+             nthreads: int = 0, gaussian: bool = False) -> None:
     """f64-position variant: (n,4) float64 agents, (h,w) float32 trail, in place."""
     assert agents.dtype == np.float64 and agents.flags.c_contiguous
     assert trail.dtype == np.float32 and trail.flags.c_contiguous
     op = params(p)
-    rc = lib().orc_step_p64(trail.shape[1], trail.shape[0], agents.shape[0],
-                            agents.ctypes.data_as(_D), _f(trail), ctypes.byref(op),
-                            STORAGE[storage], n_steps, nthreads)
+    lib().orc_set_gaussian(1 if gaussian else 0)
+    try:
+        rc = lib().orc_step_p64(trail.shape[1], trail.shape[0], agents.shape[0],
+                                agents.ctypes.data_as(_D), _f(trail), ctypes.byref(op),
+                                STORAGE[storage], n_steps, nthreads)
+    finally:
+        lib().orc_set_gaussian(0)
     if rc != 0:
         raise RuntimeError(f"orc_step_p64 failed: {rc}")
 

@@ -385,6 +385,83 @@ void orc_trail_pass(const float* tin, const uint8_t* mark, uint32_t w, uint32_t 
   free(t2);
 }
 
+/* ---- Gaussian diffuse (the library's SLM_FLAG_GAUSSIAN_DIFFUSE extension) ----
+ * Not in the reference: it carries blur_radius and blur_sigma (src/main.rs:44-45,
+ * src/settings.rs:24-25) but its shader only runs the 3x3 box.  Radius
+ * R = round(blur_radius) in 1..8, sigma > 0; weights w[i] = exp(-i^2/(2 sigma^2))
+ * normalised over i = -R..R in double and rounded to float.  Returns R, or 0
+ * (all weights 0) when the fields do not describe such a Gaussian. */
+int orc_gauss_weights(float radius, float sigma, float* w) {
+  for (int i = 0; i <= 8; i++) w[i] = 0.0f;
+  if (!(radius >= 0.5f && radius < 8.5f) || !(sigma > 0.0f) || !isfinite(sigma)) return 0;
+  int r = (int)floor((double)radius + 0.5);
+  double s2 = 2.0 * (double)sigma * sigma, e[9], sum = 0.0;
+  for (int i = 0; i <= r; i++) {
+    e[i] = exp(-(double)(i * i) / s2);
+    sum += i == 0 ? e[i] : 2.0 * e[i];
+  }
+  for (int i = 0; i <= r; i++) w[i] = (float)(e[i] / sum);
+  return r;
+}
+
+/* Deposit and decay as orc_trail_pass, then H = horizontal taps x+i, i = -R..R
+ * ascending, G = vertical taps y+j of H likewise (toroidal), one rounding per
+ * product and add, and out = mix(T2, G, clamp(rate, 0, 1)). */
+void orc_trail_pass_gauss(const float* tin, const uint8_t* mark, uint32_t w, uint32_t h,
+                          const orc_params* p, int storage, float* tout, int nthreads) {
+  uint64_t cells = (uint64_t)w * h;
+  float* t2 = (float*)malloc(cells * sizeof(float));
+  float* hs = (float*)malloc(cells * sizeof(float));
+  float wt[9];
+  int r = orc_gauss_weights(p->blur_radius, p->blur_sigma, wt);
+  float k = p->decay_factor * 0.001f;
+  float dep = p->pheromone_deposition_amount;
+  float rate = wgsl_clamp(p->diffusion_rate, 0.0f, 1.0f);
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t c = 0; c < (int64_t)cells; c++) {
+    float v = tin[c];
+    if (mark && mark[c]) v = wgsl_clamp(v + dep, 0.0f, 1.0f);
+    t2[c] = wgsl_max(v - k, 0.0f);
+  }
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t y = 0; y < (int64_t)h; y++)
+    for (int64_t x = 0; x < (int64_t)w; x++) {
+      float s = 0.0f;
+      for (int i = -r; i <= r; i++) {
+        int64_t nx = ((x + i) % (int64_t)w + (int64_t)w) % (int64_t)w;
+        float term = wt[i < 0 ? -i : i] * t2[y * w + nx];
+        s = i == -r ? term : s + term;
+      }
+      hs[y * w + x] = s;
+    }
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t y = 0; y < (int64_t)h; y++)
+    for (int64_t x = 0; x < (int64_t)w; x++) {
+      float g = 0.0f;
+      for (int j = -r; j <= r; j++) {
+        int64_t ny = ((y + j) % (int64_t)h + (int64_t)h) % (int64_t)h;
+        float term = wt[j < 0 ? -j : j] * hs[ny * w + x];
+        g = j == -r ? term : g + term;
+      }
+      float out = orc_mix(t2[y * w + x], g, rate);
+      tout[y * w + x] = orc_round_storage_at(out, storage, (uint32_t)(y * w + x));
+    }
+  free(t2);
+  free(hs);
+}
+
+static int g_gaussian = 0;
+void orc_set_gaussian(int on) { g_gaussian = on != 0; }
+
+static void trail_pass_mode(const float* tin, const uint8_t* mark, uint32_t w, uint32_t h,
+                            const orc_params* p, int storage, float* tout, int nthreads) {
+  if (g_gaussian)
+    orc_trail_pass_gauss(tin, mark, w, h, p, storage, tout, nthreads);
+  else
+    orc_trail_pass(tin, mark, w, h, p, storage, tout, nthreads);
+}
+
 /* The trail pass in double precision: the FP64 reference for the 16-bit
  * storage modes (same deposit marks, decay, 3x3 toroidal box and mix). */
 void orc_trail_pass_f64(const double* tin, const uint8_t* mark, uint32_t w, uint32_t h,
@@ -437,7 +514,7 @@ int orc_step(uint32_t w, uint32_t h, uint64_t n, float* ag, float* trail, const 
         mark[d] = 1;
       }
     }
-    orc_trail_pass(trail, mark, w, h, p, storage, next, nt);
+    trail_pass_mode(trail, mark, w, h, p, storage, next, nt);
     memcpy(trail, next, cells * sizeof(float));
   }
   free(mark);
@@ -467,7 +544,7 @@ int orc_step_p64(uint32_t w, uint32_t h, uint64_t n, double* ag, float* trail,
         mark[d] = 1;
       }
     }
-    orc_trail_pass(trail, mark, w, h, p, storage, next, nt);
+    trail_pass_mode(trail, mark, w, h, p, storage, next, nt);
     memcpy(trail, next, cells * sizeof(float));
   }
   free(mark);

@@ -58,6 +58,10 @@ void orc_sincos(float x, float* s, float* c);
  * library's sinf/cosf (1): a stand-in for a reference whose transcendental
  * bits differ, for statistical (not bitwise) comparisons. */
 void orc_set_libm_trig(int on);
+/* The Gaussian diffuse extension: weights (returns the radius, 0 if invalid),
+ * one pass, and a switch making orc_step / orc_step_p64 use it (default 0). */
+int orc_gauss_weights(float radius, float sigma, float* w);
+void orc_set_gaussian(int on);
 float orc_wgsl_mod(float a, float b);
 float orc_fract(float v);
 float orc_mix(float a, float b, float t);
@@ -95,6 +99,9 @@ int64_t orc_agent_update(const float* trail, uint32_t w, uint32_t h, const orc_p
  * :147-161, :163-195).  mark may be NULL (no deposits). */
 void orc_trail_pass(const float* trail_in, const uint8_t* mark, uint32_t w, uint32_t h,
                     const orc_params* p, int storage, float* trail_out, int nthreads);
+/* The same pass with the Gaussian diffuse extension in place of the 3x3 box. */
+void orc_trail_pass_gauss(const float* trail_in, const uint8_t* mark, uint32_t w, uint32_t h,
+                          const orc_params* p, int storage, float* trail_out, int nthreads);
 
 /* The trail pass in double precision (FP64 reference for the storage modes). */
 void orc_trail_pass_f64(const double* trail_in, const uint8_t* mark, uint32_t w, uint32_t h,

@@ -88,7 +88,8 @@ constexpr uint64_t kMaxTotalBins = 1ull << 30;
 constexpr float kMaxTrailValue = 0x1p100f;        // keeps every 3x3 window sum finite
 constexpr uint32_t kTargetWaves = 8192;           // trail-pass units per pool
 constexpr uint32_t kKnownFlags = SLM_FLAG_NO_XCD_REMAP | SLM_FLAG_F64_POSITIONS |
-                                 SLM_FLAG_TILED_SENSING | SLM_FLAG_STOCHASTIC_ROUNDING;
+                                 SLM_FLAG_TILED_SENSING | SLM_FLAG_STOCHASTIC_ROUNDING |
+                                 SLM_FLAG_GAUSSIAN_DIFFUSE;
 
 bool finite_params(const slm_params& p) {
   const float* f = &p.decay_factor;
@@ -97,11 +98,16 @@ bool finite_params(const slm_params& p) {
   return true;
 }
 
-int check_params(const slm_params* p) {
+// `flags`: the handle's SLM_FLAG_* (the Gaussian diffuse reads the blur fields).
+int check_params(const slm_params* p, uint32_t flags) {
   if (!p) return fail(SLM_ERR_INVALID_ARGUMENT, "params is NULL");
   if (!finite_params(*p)) return fail(SLM_ERR_INVALID_ARGUMENT, "params must be finite");
   if (std::fabs(p->agent_sensor_distance) > kMaxSensorDistance)
     return fail(SLM_ERR_INVALID_ARGUMENT, "|agent_sensor_distance| must be <= 2^22");
+  float w[kGaussMaxR + 1];
+  if ((flags & SLM_FLAG_GAUSSIAN_DIFFUSE) && gauss_weights(p->blur_radius, p->blur_sigma, w) == 0)
+    return fail(SLM_ERR_INVALID_ARGUMENT,
+                "Gaussian diffuse needs round(blur_radius) in 1..%d and blur_sigma > 0", kGaussMaxR);
   return SLM_OK;
 }
 
@@ -327,6 +333,8 @@ class Engine {
     p.preload_rows = preload_rows_;
     p.stochastic_rounding = (cfg.flags & SLM_FLAG_STOCHASTIC_ROUNDING) ? 1 : 0;
     p.streaming = static_cast<int>(cfg.streaming);
+    p.gaussian = (cfg.flags & SLM_FLAG_GAUSSIAN_DIFFUSE) ? 1 : 0;
+    p.total_gtiles = total_gtiles_;
     return p;
   }
 
@@ -1015,6 +1023,7 @@ class Engine {
   double* mass_out_ = nullptr;
   uint32_t total_agents_ = 0, total_units_ = 0, total_bins_ = 0;
   uint32_t total_tiles_ = 0, max_items_ = 0;
+  uint32_t total_gtiles_ = 0;  // Gaussian-pass tiles
   uint32_t preload_rows_ = 0;
   uint32_t* tile_inst_d_ = nullptr;
   WorkItem* items_d_ = nullptr;
@@ -1208,11 +1217,16 @@ class Engine {
       bins += in.nbins;
       plan_tiles(in, cl, kTileLog2);
     }
-    uint64_t tiles = 0;
+    uint64_t tiles = 0, gtiles = 0;
     for (auto& in : inst) {
       in.tile_off = static_cast<uint32_t>(tiles);
       tiles += in.tiles;
+      in.gtx = (in.width + kGaussTW - 1u) / kGaussTW;
+      in.gtile_off = static_cast<uint32_t>(gtiles);
+      gtiles += static_cast<uint64_t>(in.gtx) * ((in.height + kGaussTH - 1u) / kGaussTH);
     }
+    if (gtiles >= (1ull << 31)) return fail(SLM_ERR_INVALID_ARGUMENT, "too many trail tiles");
+    total_gtiles_ = static_cast<uint32_t>(gtiles);
     tiles_ready_ = false;
     if (units >= (1ull << 32)) return fail(SLM_ERR_INVALID_ARGUMENT, "too many trail units");
     if (sorting() && bins > kMaxTotalBins)
@@ -1338,6 +1352,7 @@ class Engine {
     uint32_t k0 = 0, k1 = 0;  // instances [k0, k1)
     uint32_t slot_base = 0, agents = 0, unit_base = 0, units = 0;
     uint64_t trail_base = 0;
+    uint32_t gtile_base = 0, gtiles = 0;  // Gaussian-pass tiles
     uint32_t preload_rows = 0;
     const Instance* inst_d = nullptr;  // tables inside group_tables_
     const uint32_t* agent_block_d = nullptr;
@@ -1414,12 +1429,15 @@ class Engine {
       G.unit_base = a.unit_off;
       G.units = z.unit_off + z.units - a.unit_off;
       G.trail_base = a.trail_off;
+      G.gtile_base = a.gtile_off;
+      G.gtiles = z.gtile_off + z.gtx * ((z.height + kGaussTH - 1u) / kGaussTH) - a.gtile_off;
       std::vector<Instance> ins(inst.begin() + G.k0, inst.begin() + G.k1);
       bool first = true;
       for (Instance& in : ins) {
         in.agent_off -= G.slot_base;
         in.unit_off -= G.unit_base;
         in.trail_off -= G.trail_base;
+        in.gtile_off -= G.gtile_base;
         if (in.ntx == 0) continue;
         uint32_t r = in.tile_rows == 4u || in.tile_rows == 8u || in.tile_rows == 16u ? in.tile_rows : 0u;
         G.preload_rows = first || G.preload_rows == r ? r : 0u;
@@ -1466,6 +1484,7 @@ class Engine {
     p.agent_block_inst = g.agent_block_d;
     p.unit_block_inst = g.unit_block_d;
     p.preload_rows = g.preload_rows;
+    p.total_gtiles = g.gtiles;
     p.total_bins = 0;
     p.total_tiles = 0;
     p.max_items = 0;
@@ -1850,7 +1869,7 @@ int slm_create(const slm_config* cfg, const slm_params* params, slm_sim** out) {
   if (!out) return fail(SLM_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
   if (!cfg) return fail(SLM_ERR_INVALID_ARGUMENT, "cfg is NULL");
-  SLM_TRY(check_params(params));
+  SLM_TRY(check_params(params, cfg->flags));
   SLM_TRY(check_grid(cfg->width, cfg->height));
   if (cfg->agent_count > kMaxAgents)
     return fail(SLM_ERR_INVALID_ARGUMENT, "agent_count must be < 2^32 - 256");
@@ -1897,7 +1916,7 @@ int slm_destroy(slm_sim* s) {
 int slm_set_params(slm_sim* s, const slm_params* p) {
   SLM_ENTRY_BEGIN
   SIM_GUARD(s);
-  SLM_TRY(check_params(p));
+  SLM_TRY(check_params(p, s->eng->cfg.flags));
   return s->eng->set_params(0, *p);
   SLM_ENTRY_END
 }
@@ -2126,6 +2145,7 @@ int slm_load_snapshot(slm_sim* s, const void* buf, uint64_t bytes) {
       ((hd.flags & SLM_FLAG_F64_POSITIONS) != 0) != e.pos64())
     return fail(SLM_ERR_INVALID_ARGUMENT,
                 "snapshot shape (grid, agents, storage, positions) differs from the simulation's");
+  SLM_TRY(check_params(&hd.params, e.cfg.flags));
   return s->eng->load_snapshot(static_cast<const uint8_t*>(buf));
   SLM_ENTRY_END
 }
@@ -2194,7 +2214,7 @@ int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts
   for (uint32_t k = 0; k < count; k++) {
     const slm_instance_desc& d = insts[k];
     int rc = check_grid(d.width, d.height);
-    if (!rc) rc = check_params(&d.params);
+    if (!rc) rc = check_params(&d.params, cfg->flags);
     if (rc) return fail(rc, "instance %u: %s", k, g_last_error);
     agents += d.agent_count;
     cells += static_cast<uint64_t>(d.width) * d.height;
@@ -2263,7 +2283,7 @@ int slm_batch_set_params(slm_batch* b, uint32_t k, const slm_params* p) {
   SLM_ENTRY_BEGIN
   BATCH_GUARD(b);
   BATCH_INDEX(b, k);
-  SLM_TRY(check_params(p));
+  SLM_TRY(check_params(p, b->eng->cfg.flags));
   return b->eng->set_params(k, *p);
   SLM_ENTRY_END
 }

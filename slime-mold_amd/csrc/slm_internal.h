@@ -26,7 +26,18 @@ struct DevParams {
   float deposit;         // pheromone_deposition_amount
   float blur_radius;
   float blur_sigma;
+  // Gaussian diffuse mode: integer radius round(blur_radius) in 1..kGaussMaxR
+  // (0 when blur_radius / blur_sigma do not describe one) and the normalised
+  // weights w[i] = exp(-i^2 / (2 sigma^2)) / sum, i = 0..radius, computed in
+  // double and rounded to f32.
+  int gauss_r;
+  float gauss_w[9];
 };
+constexpr int kGaussMaxR = 8;
+
+// Gaussian weights for blur_radius / blur_sigma; returns the radius, or 0 if
+// they do not describe a Gaussian of radius 1..kGaussMaxR.
+int gauss_weights(float blur_radius, float blur_sigma, float* w);
 
 DevParams make_dev_params(const slm_params& p);
 
@@ -59,6 +70,7 @@ struct Instance {
   uint32_t nbx, nby;                // bins across / down
   uint32_t tile_log2;               // sensing tiles of 2^k x 2^k cells (aligned bin squares)
   uint32_t tile_off, tiles;         // tiles [tile_off, tile_off + tiles), Morton order
+  uint32_t gtile_off, gtx;          // Gaussian-pass tiles [gtile_off, ...), gtx across
 };
 
 // One workgroup's share of the tiled agent pass: agent slots [begin, end) of
@@ -93,7 +105,12 @@ struct Pool {
   uint32_t preload_rows;       // 4, 8 or 16 when every strip instance has tiles of that height, else 0
   int stochastic_rounding;     // bf16 trail-pass stores round stochastically (deterministic hash)
   int streaming;               // trail-pass rows: 1 non-temporal stores, 2 non-temporal loads
+  int gaussian;                // trail pass diffuses with the Gaussian, not the 3x3 box
+  uint32_t total_gtiles;       // Gaussian-pass tiles of the pool
 };
+
+// Gaussian-pass tiles: kGaussTW columns x kGaussTH rows per workgroup.
+constexpr uint32_t kGaussTW = 256, kGaussTH = 16;
 
 constexpr uint32_t kTileLog2 = 6;  // sensing tiles of 64 x 64 cells
 

@@ -17,6 +17,8 @@
 // and below in registers; the left/right neighbour cells come from the
 // adjacent lanes, and only lanes at a wave edge or at the wrap load a halo
 // cell.  Rows of other lengths use 64-cell units, one cell per lane.
+#include <cmath>
+
 #include "slm_internal.h"
 
 namespace slm {
@@ -563,6 +565,106 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
     trail_kernel<S, 0><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
 }
 
+
+// ---- Gaussian diffuse (SLM_FLAG_GAUSSIAN_DIFFUSE) ------------------------------
+// One 256-thread workgroup per tile of kGaussTW x kGaussTH cells.  The tile's
+// deposited and decayed cells plus an R-cell toroidal halo on every side are
+// staged in LDS (thread c loads column c, and column c + 256 of the halo), the
+// horizontal pass writes a second LDS array (rows of the halo included), and
+// the vertical pass reads it down the thread's column.  Both passes sum in
+// ascending tap order, one rounding per product and per add, so the result is
+// the oracle's bit for bit.  LDS is sized for the largest radius, so a
+// captured graph stays valid when slm_set_params changes the radius.
+struct GaussArgs {
+  const void* in;
+  void* out;
+  uint32_t count;  // instances
+};
+
+constexpr int kGaussRows = static_cast<int>(kGaussTH) + 2 * kGaussMaxR;
+constexpr int kGaussCols = static_cast<int>(kGaussTW) + 2 * kGaussMaxR;
+
+// v mod n for |v| < 2^25 and 1 <= n <= 2^24 (grid sides are at most 2^24).
+__device__ inline uint32_t wrap_index(int32_t v, uint32_t n) {
+  int32_t m = v % static_cast<int32_t>(n);
+  return static_cast<uint32_t>(m < 0 ? m + static_cast<int32_t>(n) : m);
+}
+
+template <int S>
+__global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
+                                                         const Instance* __restrict__ insts,
+                                                         const DevParams* __restrict__ params) {
+  __shared__ float s_t[kGaussRows][kGaussCols];
+  __shared__ float s_h[kGaussRows][kGaussTW];
+  using elem = typename Codec<S>::elem;
+  const uint32_t t = blockIdx.x;
+  uint32_t lo = 0, hi = a.count - 1u;  // last instance whose tiles start at or before t
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1u) >> 1;
+    if (insts[mid].gtile_off <= t)
+      lo = mid;
+    else
+      hi = mid - 1u;
+  }
+  const Instance in = insts[lo];
+  const DevParams p = params[lo];
+  // 1..kGaussMaxR (the host checks it); clamped so LDS indices stay in bounds
+  const int R = min(max(p.gauss_r, 0), kGaussMaxR);
+  const uint32_t W = in.width, H = in.height;
+  const uint32_t lt = t - in.gtile_off;
+  const uint32_t x0 = (lt % in.gtx) * kGaussTW, y0 = (lt / in.gtx) * kGaussTH;
+  const elem* src = static_cast<const elem*>(a.in) + in.trail_off;
+  elem* dst = static_cast<elem*>(a.out) + in.trail_off;
+  const int c = static_cast<int>(threadIdx.x);
+  const int rows = static_cast<int>(kGaussTH) + 2 * R;
+  const int cols = static_cast<int>(kGaussTW) + 2 * R;
+  const int32_t ix0 = static_cast<int32_t>(x0), iy0 = static_cast<int32_t>(y0);
+  const uint32_t gx0 = wrap_index(ix0 + c - R, W);
+  const bool second = c + static_cast<int>(kGaussTW) < cols;
+  const uint32_t gx1 = wrap_index(ix0 + c + static_cast<int32_t>(kGaussTW) - R, W);
+  uint32_t gy = wrap_index(iy0 - R, H);
+  for (int r = 0; r < rows; r++) {
+    const elem* row = src + static_cast<uint64_t>(gy) * W;
+    s_t[r][c] = deposit_decay<S>(row[gx0], p);
+    if (second) s_t[r][c + kGaussTW] = deposit_decay<S>(row[gx1], p);
+    gy = gy + 1u == H ? 0u : gy + 1u;
+  }
+  __syncthreads();
+  for (int r = 0; r < rows; r++) {
+    float sum = p.gauss_w[R] * s_t[r][c];
+    for (int i = 1; i <= 2 * R; i++) {
+      int d = i - R;
+      float term = p.gauss_w[d < 0 ? -d : d] * s_t[r][c + i];
+      sum = sum + term;
+    }
+    s_h[r][c] = sum;
+  }
+  __syncthreads();
+  const uint32_t x = x0 + static_cast<uint32_t>(c);
+  if (x >= W) return;
+  for (int r = 0; r < static_cast<int>(kGaussTH); r++) {
+    const uint32_t y = y0 + static_cast<uint32_t>(r);
+    if (y >= H) break;
+    float g = p.gauss_w[R] * s_h[r][c];
+    for (int j = 1; j <= 2 * R; j++) {
+      int d = j - R;
+      float term = p.gauss_w[d < 0 ? -d : d] * s_h[r + j][c];
+      g = g + term;
+    }
+    float centre = s_t[r + R][c + R];
+    float l = centre * p.one_minus_rate;
+    float rr = g * p.rate;
+    const uint32_t cell = y * W + x;
+    dst[cell] = static_cast<elem>(encode_out<S>(l + rr, cell));
+  }
+}
+
+template <int S>
+void launch_gauss_t(const Pool& pool, const void* in, void* out, hipStream_t stream) {
+  GaussArgs a{in, out, pool.count};
+  gauss_kernel<S><<<pool.total_gtiles, kGaussTW, 0, stream>>>(a, pool.inst, pool.params);
+}
+
 }  // namespace
 
 void plan_trail(Instance& in, int storage, uint32_t target_waves, uint32_t fixed_rows) {
@@ -599,10 +701,45 @@ DevParams make_dev_params(const slm_params& s) {
   d.deposit = s.pheromone_deposition_amount;
   d.blur_radius = s.blur_radius;
   d.blur_sigma = s.blur_sigma;
+  d.gauss_r = gauss_weights(s.blur_radius, s.blur_sigma, d.gauss_w);
   return d;
 }
 
+int gauss_weights(float blur_radius, float blur_sigma, float* w) {
+  for (int i = 0; i <= kGaussMaxR; i++) w[i] = 0.0f;
+  if (!(blur_radius >= 0.5f && blur_radius < kGaussMaxR + 0.5f) || !(blur_sigma > 0.0f) ||
+      !std::isfinite(blur_sigma))
+    return 0;
+  const int r = static_cast<int>(std::floor(static_cast<double>(blur_radius) + 0.5));
+  const double s2 = 2.0 * static_cast<double>(blur_sigma) * blur_sigma;
+  double e[kGaussMaxR + 1];
+  double sum = 0.0;
+  for (int i = 0; i <= r; i++) {
+    e[i] = std::exp(-static_cast<double>(i * i) / s2);
+    sum += i == 0 ? e[i] : 2.0 * e[i];
+  }
+  for (int i = 0; i <= r; i++) w[i] = static_cast<float>(e[i] / sum);
+  return r;
+}
+
 void launch_trail(const Pool& pool, const void* in, void* out, hipStream_t stream) {
+  if (pool.gaussian) {
+    if (pool.total_gtiles == 0) return;
+    switch (pool.storage) {
+      case SLM_STORAGE_BF16:
+        if (pool.stochastic_rounding)
+          launch_gauss_t<kStorageBf16Sr>(pool, in, out, stream);
+        else
+          launch_gauss_t<SLM_STORAGE_BF16>(pool, in, out, stream);
+        break;
+      case SLM_STORAGE_F16:
+        launch_gauss_t<SLM_STORAGE_F16>(pool, in, out, stream);
+        break;
+      default:
+        launch_gauss_t<SLM_STORAGE_F32>(pool, in, out, stream);
+    }
+    return;
+  }
   if (pool.total_units == 0) return;
   switch (pool.storage) {
     case SLM_STORAGE_BF16:

@@ -27,6 +27,7 @@ FLAG_NO_XCD_REMAP = 0x1
 FLAG_F64_POSITIONS = 0x2
 FLAG_TILED_SENSING = 0x4
 FLAG_STOCHASTIC_ROUNDING = 0x8
+FLAG_GAUSSIAN_DIFFUSE = 0x10
 
 
 class NativeLibraryMissing(RuntimeError):

@@ -32,7 +32,8 @@ class Batch:
     def __init__(self, instances: Sequence[InstanceSpec], storage: str = "f32",
                  resort_interval: int = 0, graph_chunk: int = 100, sort_cell_log2: int = 4,
                  device: int = 0, xcd_remap: bool = True, f64_positions: bool = False,
-                 tiled_sensing: bool = False, stochastic_rounding: bool = False):
+                 tiled_sensing: bool = False, stochastic_rounding: bool = False,
+                 gaussian: bool = False):
         cfg = N.default_batch_config()
         cfg.storage = N.STORAGE_NAMES[storage]
         cfg.device = device
@@ -42,7 +43,8 @@ class Batch:
         cfg.flags = ((0 if xcd_remap else N.FLAG_NO_XCD_REMAP)
                      | (N.FLAG_F64_POSITIONS if f64_positions else 0)
                      | (N.FLAG_TILED_SENSING if tiled_sensing else 0)
-                     | (N.FLAG_STOCHASTIC_ROUNDING if stochastic_rounding else 0))
+                     | (N.FLAG_STOCHASTIC_ROUNDING if stochastic_rounding else 0)
+                     | (N.FLAG_GAUSSIAN_DIFFUSE if gaussian else 0))
         descs = (N.InstanceDesc * len(instances))()
         for d, spec in zip(descs, instances):
             d.width, d.height = spec.width, spec.height

@@ -48,7 +48,8 @@ class Simulation:
                  params: Union[Settings, "N.Params", None] = None, storage: str = "f32",
                  resort_interval: int = 0, graph_chunk: int = 100, sort_cell_log2: int = 4,
                  device: int = 0, xcd_remap: bool = True, f64_positions: bool = False,
-                 tiled_sensing: bool = False, stochastic_rounding: bool = False):
+                 tiled_sensing: bool = False, stochastic_rounding: bool = False,
+                 gaussian: bool = False):
         cfg = N.default_config()
         cfg.width = width
         cfg.height = height
@@ -62,7 +63,8 @@ class Simulation:
         cfg.flags = ((0 if xcd_remap else N.FLAG_NO_XCD_REMAP)
                      | (N.FLAG_F64_POSITIONS if f64_positions else 0)
                      | (N.FLAG_TILED_SENSING if tiled_sensing else 0)
-                     | (N.FLAG_STOCHASTIC_ROUNDING if stochastic_rounding else 0))
+                     | (N.FLAG_STOCHASTIC_ROUNDING if stochastic_rounding else 0)
+                     | (N.FLAG_GAUSSIAN_DIFFUSE if gaussian else 0))
         self._params = _params(params)
         h = ctypes.c_void_p()
         N.check(N.lib().slm_create(ctypes.byref(cfg), ctypes.byref(self._params), ctypes.byref(h)))

@@ -291,3 +291,12 @@ def test_ctypes_mirror_matches_the_c_layout(slm):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for fname, _ in cls._fields_:
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_python_flag_constants_match_the_header(slm):
+    import slimemold._native as N
+    text = open(HEADER).read()
+    defs = dict(re.findall(r"#define SLM_(FLAG_[A-Z0-9_]+) (0x[0-9a-f]+)u", text))
+    assert len(defs) >= 5
+    for name, value in defs.items():
+        assert getattr(N, name) == int(value, 16), name

@@ -1,0 +1,115 @@
+"""The Gaussian diffuse extension (SLM_FLAG_GAUSSIAN_DIFFUSE) on the GPU against
+the oracle, bit for bit: grids smaller than the kernel and than one tile,
+ragged tile edges, radii 1..8, every storage format, re-sorting, graph chunks,
+batches with a radius per instance and instance groups, a radius changed
+under captured graphs; and the argument checks."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from parity_util import assert_bitwise, preset
+
+pytestmark = pytest.mark.gpu
+
+
+def storage_kw(storage):
+    sr = storage == "bf16sr"
+    return dict(storage="bf16" if sr else storage, stochastic_rounding=sr)
+
+
+CASES = [
+    # w, h, agents, radius, sigma, rate, storage, resort, chunk, steps
+    (1, 1, 3, 2.0, 1.0, 1.0, "f32", 0, 0, 3),
+    (3, 5, 20, 4.0, 1.5, 0.8, "f32", 0, 0, 4),
+    (257, 17, 2000, 2.0, 1.0, 1.0, "f32", 5, 4, 9),
+    (300, 70, 5000, 3.0, 2.0, 0.35, "bf16", 0, 0, 6),
+    (600, 40, 9000, 8.0, 3.0, 1.0, "f16", 3, 0, 5),
+    (512, 512, 40000, 2.0, 1.0, 1.0, "bf16sr", 10, 16, 20),
+    (1000, 33, 6000, 1.0, 0.6, 0.9, "f32", 0, 5, 7),
+    (64, 300, 4000, 5.0, 2.2, 1.0, "bf16", 2, 3, 8),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_gaussian_single_matches_oracle(slm, orc, case):
+    w, h, n, radius, sigma, rate, storage, resort, chunk, steps = CASES[case]
+    s = dataclasses.replace(preset(slm, "Default"), blur_radius=radius, blur_sigma=sigma,
+                            pheromone_diffusion_rate=rate)
+    print(f"[gauss {case}] {w}x{h} n={n} r={radius} sigma={sigma} rate={rate} {storage} "
+          f"resort={resort} chunk={chunk} steps={steps}", flush=True)
+    rng = np.random.default_rng(case)
+    with slm.Simulation(w, h, n, seed=case + 11, params=s, resort_interval=resort,
+                        graph_chunk=chunk, gaussian=True, **storage_kw(storage)) as sim:
+        sim.write_trail(rng.uniform(0.0, 1.0, (h, w)).astype(np.float32))
+        a, t = sim.read_agents(), sim.read_trail()
+        sim.step(steps)
+        ga, gt = sim.read_agents(), sim.read_trail()
+    orc.step(a, t, s, steps, storage=storage, gaussian=True)
+    assert_bitwise(ga, a, f"case {case} agents")
+    assert_bitwise(gt, t, f"case {case} trail")
+
+
+@pytest.mark.parametrize("groups", [1, 3])
+def test_gaussian_batch_radius_per_instance(slm, orc, groups):
+    rng = np.random.default_rng(40 + groups)
+    specs = []
+    for k in range(7):
+        w, h = int(rng.integers(1, 700)), int(rng.integers(1, 300))
+        s = dataclasses.replace(preset(slm, ["Default", "Sponge", "Threads"][k % 3]),
+                                blur_radius=float(1 + k % 8), blur_sigma=float(rng.uniform(0.4, 3.0)),
+                                pheromone_diffusion_rate=float(rng.uniform(0.2, 1.0)))
+        specs.append(slm.InstanceSpec(w, h, int(rng.integers(0, 3 * w * h // 2 + 2)),
+                                      seed=100 + k, params=s))
+    print(f"[gauss batch] groups={groups} {[(sp.width, sp.height, sp.agent_count) for sp in specs]}",
+          flush=True)
+    with slm.Batch(specs, graph_chunk=4, resort_interval=3, gaussian=True, storage="bf16") as b:
+        b.set_tuning(instance_groups=groups)
+        starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+        b.step(9)
+        got = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+    for k, sp in enumerate(specs):
+        a, t = starts[k]
+        orc.step(a, t, sp.params, 9, storage="bf16", gaussian=True)
+        assert_bitwise(got[k][0], a, f"instance {k} agents")
+        assert_bitwise(got[k][1], t, f"instance {k} trail")
+
+
+def test_gaussian_radius_change_under_captured_graphs(slm, orc):
+    s2 = dataclasses.replace(preset(slm, "Default"), blur_radius=2.0, blur_sigma=1.0)
+    s8 = dataclasses.replace(s2, blur_radius=8.0, blur_sigma=4.0)
+    print("[gauss radius change] 400x200 n=20000 chunk=5", flush=True)
+    with slm.Simulation(400, 200, 20000, seed=5, params=s2, graph_chunk=5, gaussian=True) as sim:
+        a, t = sim.read_agents(), sim.read_trail()
+        sim.step(10)  # graphs captured with radius 2
+        sim.set_params(s8)
+        sim.step(10)  # replayed with radius 8
+        ga, gt = sim.read_agents(), sim.read_trail()
+    orc.step(a, t, s2, 10, gaussian=True)
+    orc.step(a, t, s8, 10, gaussian=True)
+    assert_bitwise(ga, a, "agents")
+    assert_bitwise(gt, t, "trail")
+
+
+def test_gaussian_params_are_checked(slm):
+    base = preset(slm, "Default")
+    for radius, sigma in [(0.0, 1.0), (0.4, 1.0), (8.6, 1.0), (2.0, 0.0), (2.0, -3.0)]:
+        bad = dataclasses.replace(base, blur_radius=radius, blur_sigma=sigma)
+        with pytest.raises(slm.SlimeMoldError):
+            slm.Simulation(64, 64, 10, params=bad, gaussian=True)
+        with slm.Simulation(64, 64, 10, params=bad) as sim:  # the box diffuse ignores them
+            sim.step(1)
+        with slm.Simulation(64, 64, 10, params=base, gaussian=True) as sim:
+            with pytest.raises(slm.SlimeMoldError):
+                sim.set_params(bad)
+            sim.step(1)  # still usable
+        with pytest.raises(slm.SlimeMoldError):
+            slm.Batch([slm.InstanceSpec(32, 32, 5, params=base),
+                       slm.InstanceSpec(32, 32, 5, params=bad)], gaussian=True)
+    # a snapshot whose parameters do not describe a Gaussian cannot be loaded into one
+    bad = dataclasses.replace(base, blur_radius=0.0)
+    with slm.Simulation(64, 64, 10, params=bad) as src:
+        snap = src.snapshot()
+    with slm.Simulation(64, 64, 10, params=base, gaussian=True) as sim:
+        with pytest.raises(slm.SlimeMoldError):
+            sim.load_snapshot(snap)
