@@ -104,6 +104,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=1000, help="batched leg steps (0: skip)")
     ap.add_argument("--diffuse-steps", type=int, default=500,
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
+    ap.add_argument("--gaussian-steps", type=int, default=300,
+                    help="steps of each Gaussian-diffuse leg, same maps (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--separate-steps", type=int, default=300,
                     help="steps of the batched leg's instances run as separate f32 "
@@ -236,17 +238,21 @@ def autotune(handle, trial_steps, label):
     return d
 
 
-def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps, rounding="nearest"):
+def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps, rounding="nearest",
+                gaussian=False):
     """The trail pass alone on a map larger than the 256 MB Infinity Cache:
     `side`^2 cells and no agents, so every step is one trail-kernel launch;
     graph chunks, wall time of `steps` steps between synchronisations (launch
     gaps included, so a lower bound on the kernel's own bandwidth), after the
-    autotuner has picked the tile rows."""
+    autotuner has picked the tile rows (the 3x3 box pass; the Gaussian pass,
+    at the Default preset's blur radius 2 and sigma 1, has no tuning)."""
     sb = STORAGE_BYTES[storage]
     sr = storage == "bf16" and rounding == "stochastic"
     sim = slm.Simulation(side, side, 0, seed=3, params=slm.workloads.preset("Default"),
-                         storage=storage, graph_chunk=graph_chunk, stochastic_rounding=sr)
-    tuning = autotune(sim, tune_steps, f"diffuse {side}^2 {storage}")
+                         storage=storage, graph_chunk=graph_chunk, stochastic_rounding=sr,
+                         gaussian=gaussian)
+    kind = "gaussian" if gaussian else "diffuse"
+    tuning = autotune(sim, 0 if gaussian else tune_steps, f"{kind} {side}^2 {storage}")
     sim.step(2 * graph_chunk)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -256,12 +262,17 @@ def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps, rounding="ne
     sim.close()
     cells = side * side
     gbps = trail_bytes(sb) * cells * steps / dt / 1e9
-    log(f"diffuse {side}^2 {storage}: {dt / steps * 1e6:.1f} us/step, {gbps:.0f} GB/s")
-    return {"side": side, "storage": storage, "map_bytes": cells * sb, "steps": steps,
-            "us_per_step": round(dt / steps * 1e6, 2), "achieved_gbps": round(gbps, 1),
-            "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
-            "cells_per_s": cells * steps / dt, "trail_rows": tuning["trail_rows"],
-            "rounding": "stochastic" if sr else "nearest"}
+    log(f"{kind} {side}^2 {storage}: {dt / steps * 1e6:.1f} us/step, {gbps:.0f} GB/s")
+    out = {"side": side, "storage": storage, "map_bytes": cells * sb, "steps": steps,
+           "us_per_step": round(dt / steps * 1e6, 2), "achieved_gbps": round(gbps, 1),
+           "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
+           "cells_per_s": cells * steps / dt, "rounding": "stochastic" if sr else "nearest"}
+    if gaussian:
+        out.update(kernel="gauss_kernel (deposit + decay + separable Gaussian)", radius=2,
+                   sigma=1.0)
+    else:
+        out["trail_rows"] = tuning["trail_rows"]
+    return out
 
 
 def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label,
@@ -452,6 +463,12 @@ def main():
                                           args.autotune),
                               diffuse_leg(slm, 16384, "bf16", args.diffuse_steps,
                                           args.graph_chunk, args.autotune, args.rounding)]
+    if args.gaussian_steps > 0:
+        out["gaussian_hbm"] = [
+            diffuse_leg(slm, 8192, "f32", args.gaussian_steps, args.graph_chunk, 0,
+                        gaussian=True),
+            diffuse_leg(slm, 16384, "bf16", args.gaussian_steps, args.graph_chunk, 0,
+                        args.rounding, gaussian=True)]
     if args.batch_steps > 0:
         specs = slm.workloads.batch_mix(args.batch_count)
         bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.batch_resort,

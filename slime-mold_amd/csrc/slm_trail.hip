@@ -569,9 +569,9 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
 // ---- Gaussian diffuse (SLM_FLAG_GAUSSIAN_DIFFUSE) ------------------------------
 // One 256-thread workgroup per tile of kGaussTW x kGaussTH cells.  The tile's
 // deposited and decayed cells plus an R-cell toroidal halo on every side are
-// staged in LDS (thread c loads column c, and column c + 256 of the halo), the
-// horizontal pass writes a second LDS array (rows of the halo included), and
-// the vertical pass reads it down the thread's column.  Both passes sum in
+// staged in LDS (thread c loads column c, and column c + 256 of the halo);
+// thread c then runs the horizontal pass for column c over every row of the
+// tile and its halo into registers, and the vertical pass from them.  Both passes sum in
 // ascending tap order, one rounding per product and per add, so the result is
 // the oracle's bit for bit.  LDS is sized for the largest radius, so a
 // captured graph stays valid when slm_set_params changes the radius.
@@ -590,12 +590,77 @@ __device__ inline uint32_t wrap_index(int32_t v, uint32_t n) {
   return static_cast<uint32_t>(m < 0 ? m + static_cast<int32_t>(n) : m);
 }
 
+// One tile at radius R (a compile-time constant, so every loop unrolls and the
+// weights sit in registers).
+template <int S, int R>
+__device__ inline void gauss_tile(float (*s_t)[kGaussCols],
+                                  const typename Codec<S>::elem* src, typename Codec<S>::elem* dst,
+                                  uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
+                                  const DevParams& p, const float* wts) {
+  constexpr int rows = static_cast<int>(kGaussTH) + 2 * R;
+  constexpr int cols = static_cast<int>(kGaussTW) + 2 * R;
+  float w[R + 1];
+#pragma unroll
+  for (int i = 0; i <= R; i++) w[i] = wts[i];
+  const int c = static_cast<int>(threadIdx.x);
+  const int32_t ix0 = static_cast<int32_t>(x0), iy0 = static_cast<int32_t>(y0);
+  const uint32_t gx0 = wrap_index(ix0 + c - R, W);
+  const bool second = c + static_cast<int>(kGaussTW) < cols;
+  const uint32_t gx1 = wrap_index(ix0 + c + static_cast<int32_t>(kGaussTW) - R, W);
+  // every row's loads are issued before any is used
+  uint32_t raw0[rows], raw1[rows];
+  uint32_t gy = wrap_index(iy0 - R, H);
+#pragma unroll
+  for (int r = 0; r < rows; r++) {
+    const typename Codec<S>::elem* row = src + static_cast<uint64_t>(gy) * W;
+    raw0[r] = row[gx0];
+    raw1[r] = second ? static_cast<uint32_t>(row[gx1]) : 0u;
+    gy = gy + 1u == H ? 0u : gy + 1u;
+  }
+#pragma unroll
+  for (int r = 0; r < rows; r++) {
+    s_t[r][c] = deposit_decay<S>(raw0[r], p);
+    if (second) s_t[r][c + kGaussTW] = deposit_decay<S>(raw1[r], p);
+  }
+  __syncthreads();
+  // the horizontal pass of the thread's own column, every row of the tile
+  // and its halo, stays in registers for the vertical pass
+  float h[rows];
+#pragma unroll
+  for (int r = 0; r < rows; r++) {
+    float sum = w[R] * s_t[r][c];
+#pragma unroll
+    for (int i = 1; i <= 2 * R; i++) {
+      float term = w[i < R ? R - i : i - R] * s_t[r][c + i];
+      sum = sum + term;
+    }
+    h[r] = sum;
+  }
+  const uint32_t x = x0 + static_cast<uint32_t>(c);
+  if (x >= W) return;
+#pragma unroll
+  for (int r = 0; r < static_cast<int>(kGaussTH); r++) {
+    const uint32_t y = y0 + static_cast<uint32_t>(r);
+    if (y >= H) break;
+    float g = w[R] * h[r];
+#pragma unroll
+    for (int j = 1; j <= 2 * R; j++) {
+      float term = w[j < R ? R - j : j - R] * h[r + j];
+      g = g + term;
+    }
+    float centre = s_t[r + R][c + R];
+    float l = centre * p.one_minus_rate;
+    float rr = g * p.rate;
+    const uint32_t cell = y * W + x;
+    dst[cell] = static_cast<typename Codec<S>::elem>(encode_out<S>(l + rr, cell));
+  }
+}
+
 template <int S>
 __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
                                                          const Instance* __restrict__ insts,
                                                          const DevParams* __restrict__ params) {
   __shared__ float s_t[kGaussRows][kGaussCols];
-  __shared__ float s_h[kGaussRows][kGaussTW];
   using elem = typename Codec<S>::elem;
   const uint32_t t = blockIdx.x;
   uint32_t lo = 0, hi = a.count - 1u;  // last instance whose tiles start at or before t
@@ -608,54 +673,22 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
   }
   const Instance in = insts[lo];
   const DevParams p = params[lo];
-  // 1..kGaussMaxR (the host checks it); clamped so LDS indices stay in bounds
-  const int R = min(max(p.gauss_r, 0), kGaussMaxR);
-  const uint32_t W = in.width, H = in.height;
   const uint32_t lt = t - in.gtile_off;
   const uint32_t x0 = (lt % in.gtx) * kGaussTW, y0 = (lt / in.gtx) * kGaussTH;
   const elem* src = static_cast<const elem*>(a.in) + in.trail_off;
   elem* dst = static_cast<elem*>(a.out) + in.trail_off;
-  const int c = static_cast<int>(threadIdx.x);
-  const int rows = static_cast<int>(kGaussTH) + 2 * R;
-  const int cols = static_cast<int>(kGaussTW) + 2 * R;
-  const int32_t ix0 = static_cast<int32_t>(x0), iy0 = static_cast<int32_t>(y0);
-  const uint32_t gx0 = wrap_index(ix0 + c - R, W);
-  const bool second = c + static_cast<int>(kGaussTW) < cols;
-  const uint32_t gx1 = wrap_index(ix0 + c + static_cast<int32_t>(kGaussTW) - R, W);
-  uint32_t gy = wrap_index(iy0 - R, H);
-  for (int r = 0; r < rows; r++) {
-    const elem* row = src + static_cast<uint64_t>(gy) * W;
-    s_t[r][c] = deposit_decay<S>(row[gx0], p);
-    if (second) s_t[r][c + kGaussTW] = deposit_decay<S>(row[gx1], p);
-    gy = gy + 1u == H ? 0u : gy + 1u;
-  }
-  __syncthreads();
-  for (int r = 0; r < rows; r++) {
-    float sum = p.gauss_w[R] * s_t[r][c];
-    for (int i = 1; i <= 2 * R; i++) {
-      int d = i - R;
-      float term = p.gauss_w[d < 0 ? -d : d] * s_t[r][c + i];
-      sum = sum + term;
-    }
-    s_h[r][c] = sum;
-  }
-  __syncthreads();
-  const uint32_t x = x0 + static_cast<uint32_t>(c);
-  if (x >= W) return;
-  for (int r = 0; r < static_cast<int>(kGaussTH); r++) {
-    const uint32_t y = y0 + static_cast<uint32_t>(r);
-    if (y >= H) break;
-    float g = p.gauss_w[R] * s_h[r][c];
-    for (int j = 1; j <= 2 * R; j++) {
-      int d = j - R;
-      float term = p.gauss_w[d < 0 ? -d : d] * s_h[r + j][c];
-      g = g + term;
-    }
-    float centre = s_t[r + R][c + R];
-    float l = centre * p.one_minus_rate;
-    float rr = g * p.rate;
-    const uint32_t cell = y * W + x;
-    dst[cell] = static_cast<elem>(encode_out<S>(l + rr, cell));
+  const float* wts = params[lo].gauss_w;
+  // 1..kGaussMaxR (the host checks it); any other value writes nothing
+  switch (p.gauss_r) {
+    case 1: gauss_tile<S, 1>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    case 2: gauss_tile<S, 2>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    case 3: gauss_tile<S, 3>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    case 4: gauss_tile<S, 4>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    case 5: gauss_tile<S, 5>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    case 6: gauss_tile<S, 6>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    case 7: gauss_tile<S, 7>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    case 8: gauss_tile<S, 8>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+    default: break;
   }
 }
 
