@@ -1,7 +1,8 @@
 // slm_trail.hip - the fused trail pass: deposit marks + decay + 3x3 box diffuse
-// (reference src/compute.wgsl:140, :147-161, :163-195), plus the small trail
-// utilities behind the C ABI (parameter upload, f32 <-> storage conversion,
-// f64 total mass).
+// (reference src/compute.wgsl:140, :147-161, :163-195), its opt-in variant
+// with a separable Gaussian in place of the box (gauss_kernel, below), plus the
+// small trail utilities behind the C ABI (parameter upload, f32 <-> storage
+// conversion, f64 total mass).
 //
 // One read and one write of the trail per cell and step (Jacobi: trail_in ->
 // trail_out).  For each cell the pass computes, in f32:
