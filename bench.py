@@ -28,7 +28,10 @@ roofline of the plain leg's dominant kernel, whose average launch time comes
 from HIP events recorded inside the captured graph chunks of a further
 segment of as many steps, same presets, run right after the timed region; the
 diffuse kernel's achieved bandwidth; the batched leg; and the CPU oracle timed
-on a bounded sample of the plain leg's state on the host cores.
+on a bounded sample of the plain leg's state on the host cores.  Beside them,
+the library's Gaussian-diffuse extension at the presets' blur radius 2 (the
+plain workload as "plain_gaussian", the pass alone as "gaussian_hbm"); the
+headline stays the reference's 3x3 box.
 """
 import argparse
 import json
@@ -106,6 +109,8 @@ def parse():
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
     ap.add_argument("--gaussian-steps", type=int, default=300,
                     help="steps of each Gaussian-diffuse leg, same maps (0: skip)")
+    ap.add_argument("--plain-gaussian-steps", type=int, default=2000,
+                    help="steps of the plain workload with the Gaussian diffuse (0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--separate-steps", type=int, default=300,
                     help="steps of the batched leg's instances run as separate f32 "
@@ -224,6 +229,32 @@ def plain_leg(slm, args):
         "trail_roof": roofline("trail_kernel", t_bytes, t_s),
         "cells": W * H, "agents": N, "state0": state0, "presets": presets, "tuning": tuning,
     }
+
+
+def plain_gaussian_leg(slm, args, steps):
+    """The plain leg's workload and tuning with the Gaussian diffuse in place of
+    the 3x3 box, at the presets' blur radius 2 and sigma 1: wall time of
+    `steps` steps (presets in thirds, as the plain leg)."""
+    presets = [slm.workloads.preset(n) for n in slm.workloads.PLAIN_PRESETS]
+    W, H, N = args.width, args.height, 1 << args.agents_log2
+    seg = [steps // 3, steps // 3, steps - 2 * (steps // 3)]
+    sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
+                         resort_interval=args.resort, graph_chunk=args.graph_chunk,
+                         sort_cell_log2=args.sort_cell_log2, gaussian=True)
+    sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
+                   streaming=args.streaming)
+    sim.step(args.warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for p, n in zip(presets, seg):
+        sim.set_params(p)
+        sim.step(n)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim.close()
+    log(f"plain gaussian: timed {steps} steps in {elapsed:.3f} s")
+    return {"agent_updates_per_s": N * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+            "steps": steps, "radius": 2, "sigma": 1.0, "storage": args.storage}
 
 
 def autotune(handle, trial_steps, label):
@@ -463,6 +494,8 @@ def main():
                                           args.autotune),
                               diffuse_leg(slm, 16384, "bf16", args.diffuse_steps,
                                           args.graph_chunk, args.autotune, args.rounding)]
+    if args.plain_gaussian_steps > 0:
+        out["plain_gaussian"] = plain_gaussian_leg(slm, args, args.plain_gaussian_steps)
     if args.gaussian_steps > 0:
         out["gaussian_hbm"] = [
             diffuse_leg(slm, 8192, "f32", args.gaussian_steps, args.graph_chunk, 0,
