@@ -130,10 +130,12 @@ typedef struct slm_config {
  * reads them).  Per instance, R = round(blur_radius) must be 1..8 and
  * blur_sigma > 0, else SLM_ERR_INVALID_ARGUMENT.  With T2 the deposited and
  * decayed trail and w[i] = exp(-i^2 / (2 sigma^2)) / sum_{|j|<=R} exp(...)
- * (double, rounded to f32), the pass computes in f32, no fused operations:
+ * (double, rounded to f32), the pass computes in f32:
  *   H[y][x] = sum_{i=-R..R} w[|i|] * T2[y][(x+i) mod W]   (i ascending)
  *   G[y][x] = sum_{j=-R..R} w[|j|] * H[(y+j) mod H][x]    (j ascending)
- *   out     = mix(T2[y][x], G[y][x], clamp(diffusion_rate, 0, 1)). */
+ *   out     = mix(T2[y][x], G[y][x], clamp(diffusion_rate, 0, 1)),
+ * each sum starting from its first product and adding every further tap with
+ * one fused multiply-add (a single rounding); mix as the 3x3 pass. */
 #define SLM_FLAG_GAUSSIAN_DIFFUSE 0x10u
 
 typedef struct slm_info {

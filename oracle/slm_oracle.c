@@ -405,8 +405,8 @@ int orc_gauss_weights(float radius, float sigma, float* w) {
 }
 
 /* Deposit and decay as orc_trail_pass, then H = horizontal taps x+i, i = -R..R
- * ascending, G = vertical taps y+j of H likewise (toroidal), one rounding per
- * product and add, and out = mix(T2, G, clamp(rate, 0, 1)). */
+ * ascending, G = vertical taps y+j of H likewise (toroidal): the first tap's
+ * product, then one fmaf per further tap; and out = mix(T2, G, clamp(rate, 0, 1)). */
 void orc_trail_pass_gauss(const float* tin, const uint8_t* mark, uint32_t w, uint32_t h,
                           const orc_params* p, int storage, float* tout, int nthreads) {
   uint64_t cells = (uint64_t)w * h;
@@ -430,8 +430,8 @@ void orc_trail_pass_gauss(const float* tin, const uint8_t* mark, uint32_t w, uin
       float s = 0.0f;
       for (int i = -r; i <= r; i++) {
         int64_t nx = ((x + i) % (int64_t)w + (int64_t)w) % (int64_t)w;
-        float term = wt[i < 0 ? -i : i] * t2[y * w + nx];
-        s = i == -r ? term : s + term;
+        float wi = wt[i < 0 ? -i : i], a = t2[y * w + nx];
+        s = i == -r ? wi * a : fmaf(wi, a, s);
       }
       hs[y * w + x] = s;
     }
@@ -441,8 +441,8 @@ void orc_trail_pass_gauss(const float* tin, const uint8_t* mark, uint32_t w, uin
       float g = 0.0f;
       for (int j = -r; j <= r; j++) {
         int64_t ny = ((y + j) % (int64_t)h + (int64_t)h) % (int64_t)h;
-        float term = wt[j < 0 ? -j : j] * hs[ny * w + x];
-        g = j == -r ? term : g + term;
+        float wj = wt[j < 0 ? -j : j], a = hs[ny * w + x];
+        g = j == -r ? wj * a : fmaf(wj, a, g);
       }
       float out = orc_mix(t2[y * w + x], g, rate);
       tout[y * w + x] = orc_round_storage_at(out, storage, (uint32_t)(y * w + x));

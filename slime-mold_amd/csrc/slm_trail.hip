@@ -572,8 +572,9 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
 // deposited and decayed cells plus an R-cell toroidal halo on every side are
 // staged in LDS (thread c loads column c, and column c + 256 of the halo);
 // thread c then runs the horizontal pass for column c over every row of the
-// tile and its halo into registers, and the vertical pass from them.  Both passes sum in
-// ascending tap order, one rounding per product and per add, so the result is
+// tile and its halo into registers, and the vertical pass from them.  Both
+// passes start from the product with the first tap's weight and add the others
+// in ascending tap order with one fused multiply-add each, so the result is
 // the oracle's bit for bit.  LDS is sized for the largest radius, so a
 // captured graph stays valid when slm_set_params changes the radius.
 struct GaussArgs {
@@ -591,37 +592,99 @@ __device__ inline uint32_t wrap_index(int32_t v, uint32_t n) {
   return static_cast<uint32_t>(m < 0 ? m + static_cast<int32_t>(n) : m);
 }
 
+// v mod n for -n <= v < 2n.
+__device__ inline uint32_t wrap_near(int32_t v, uint32_t n) {
+  const int32_t m = static_cast<int32_t>(n);
+  return static_cast<uint32_t>(v < 0 ? v + m : (v >= m ? v - m : v));
+}
+
 // One tile at radius R (a compile-time constant, so every loop unrolls and the
-// weights sit in registers).
+// weights sit in registers).  A tile whose 256 columns lie inside the row, in
+// a row of whole 16-byte chunks, moves 16 bytes per lane: the rows are loaded
+// as chunks (plus single halo cells), and the results are staged in LDS and
+// stored as chunks.  Other tiles load and store one cell per lane.  Both give
+// the same bits.
 template <int S, int R>
 __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
                                   const typename Codec<S>::elem* src, typename Codec<S>::elem* dst,
                                   uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
                                   const DevParams& p, const float* wts) {
+  using elem = typename Codec<S>::elem;
   constexpr int rows = static_cast<int>(kGaussTH) + 2 * R;
   constexpr int cols = static_cast<int>(kGaussTW) + 2 * R;
+  constexpr int V = static_cast<int>(16 / sizeof(elem));  // cells per 16-byte chunk
+  constexpr int CPR = static_cast<int>(kGaussTW) / V;     // chunks per tile row
   float w[R + 1];
 #pragma unroll
   for (int i = 0; i <= R; i++) w[i] = wts[i];
   const int c = static_cast<int>(threadIdx.x);
   const int32_t ix0 = static_cast<int32_t>(x0), iy0 = static_cast<int32_t>(y0);
-  const uint32_t gx0 = wrap_index(ix0 + c - R, W);
-  const bool second = c + static_cast<int>(kGaussTW) < cols;
-  const uint32_t gx1 = wrap_index(ix0 + c + static_cast<int32_t>(kGaussTW) - R, W);
-  // every row's loads are issued before any is used
-  uint32_t raw0[rows], raw1[rows];
-  uint32_t gy = wrap_index(iy0 - R, H);
+  const bool vec = W % V == 0 && x0 + kGaussTW <= W;  // block-uniform
+  if (vec) {
+    const bool near = H >= kGaussTH + R;  // rows y0-R .. y0+TH+R-1 wrap at most once
+    constexpr int kChunks = rows * CPR;
+    constexpr int kPer = (kChunks + static_cast<int>(kGaussTW) - 1) / static_cast<int>(kGaussTW);
+    constexpr int kHalo = rows * 2 * R;
+    constexpr int kHaloPer = (kHalo + static_cast<int>(kGaussTW) - 1) / static_cast<int>(kGaussTW);
+    uint4 raw[kPer];
+    uint32_t hraw[kHaloPer];
 #pragma unroll
-  for (int r = 0; r < rows; r++) {
-    const typename Codec<S>::elem* row = src + static_cast<uint64_t>(gy) * W;
-    raw0[r] = row[gx0];
-    raw1[r] = second ? static_cast<uint32_t>(row[gx1]) : 0u;
-    gy = gy + 1u == H ? 0u : gy + 1u;
-  }
+    for (int k = 0; k < kPer; k++) {
+      const int q = c + k * static_cast<int>(kGaussTW);
+      raw[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (q < kChunks) {
+        const int r = q / CPR, j = q % CPR;
+        const uint32_t gy = near ? wrap_near(iy0 - R + r, H) : wrap_index(iy0 - R + r, H);
+        raw[k] = *reinterpret_cast<const uint4*>(src + static_cast<uint64_t>(gy) * W + x0 + j * V);
+      }
+    }
 #pragma unroll
-  for (int r = 0; r < rows; r++) {
-    s_t[r][c] = deposit_decay<S>(raw0[r], p);
-    if (second) s_t[r][c + kGaussTW] = deposit_decay<S>(raw1[r], p);
+    for (int k = 0; k < kHaloPer; k++) {
+      const int q = c + k * static_cast<int>(kGaussTW);
+      hraw[k] = 0u;
+      if (q < kHalo) {
+        const int r = q / (2 * R), jj = q % (2 * R);
+        const int off = jj < R ? jj - R : static_cast<int>(kGaussTW) + jj - R;
+        const uint32_t gy = near ? wrap_near(iy0 - R + r, H) : wrap_index(iy0 - R + r, H);
+        hraw[k] = src[static_cast<uint64_t>(gy) * W + wrap_near(ix0 + off, W)];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int q = c + k * static_cast<int>(kGaussTW);
+      if (q < kChunks) {
+        const int r = q / CPR, j = q % CPR;
+#pragma unroll
+        for (int e = 0; e < V; e++) s_t[r][R + j * V + e] = deposit_decay<S>(Codec<S>::lane(raw[k], e), p);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kHaloPer; k++) {
+      const int q = c + k * static_cast<int>(kGaussTW);
+      if (q < kHalo) {
+        const int r = q / (2 * R), jj = q % (2 * R);
+        s_t[r][jj < R ? jj : static_cast<int>(kGaussTW) + jj] = deposit_decay<S>(hraw[k], p);
+      }
+    }
+  } else {
+    const uint32_t gx0 = wrap_index(ix0 + c - R, W);
+    const bool second = c + static_cast<int>(kGaussTW) < cols;
+    const uint32_t gx1 = wrap_index(ix0 + c + static_cast<int32_t>(kGaussTW) - R, W);
+    // every row's loads are issued before any is used
+    uint32_t raw0[rows], raw1[rows];
+    uint32_t gy = wrap_index(iy0 - R, H);
+#pragma unroll
+    for (int r = 0; r < rows; r++) {
+      const elem* row = src + static_cast<uint64_t>(gy) * W;
+      raw0[r] = row[gx0];
+      raw1[r] = second ? static_cast<uint32_t>(row[gx1]) : 0u;
+      gy = gy + 1u == H ? 0u : gy + 1u;
+    }
+#pragma unroll
+    for (int r = 0; r < rows; r++) {
+      s_t[r][c] = deposit_decay<S>(raw0[r], p);
+      if (second) s_t[r][c + kGaussTW] = deposit_decay<S>(raw1[r], p);
+    }
   }
   __syncthreads();
   // the horizontal pass of the thread's own column, every row of the tile
@@ -631,13 +694,40 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
   for (int r = 0; r < rows; r++) {
     float sum = w[R] * s_t[r][c];
 #pragma unroll
-    for (int i = 1; i <= 2 * R; i++) {
-      float term = w[i < R ? R - i : i - R] * s_t[r][c + i];
-      sum = sum + term;
-    }
+    for (int i = 1; i <= 2 * R; i++) sum = __builtin_fmaf(w[i < R ? R - i : i - R], s_t[r][c + i], sum);
     h[r] = sum;
   }
   const uint32_t x = x0 + static_cast<uint32_t>(c);
+  if (vec) {
+    uint32_t o[kGaussTH];
+#pragma unroll
+    for (int r = 0; r < static_cast<int>(kGaussTH); r++) {
+      const uint32_t y = y0 + static_cast<uint32_t>(r);
+      float g = w[R] * h[r];
+#pragma unroll
+      for (int j = 1; j <= 2 * R; j++) g = __builtin_fmaf(w[j < R ? R - j : j - R], h[r + j], g);
+      float l = s_t[r + R][c + R] * p.one_minus_rate;
+      float rr = g * p.rate;
+      o[r] = y < H ? encode_out<S>(l + rr, y * W + x) : 0u;
+    }
+    __syncthreads();  // every read of s_t is done: it becomes the output stage
+    elem* s_o = reinterpret_cast<elem*>(&s_t[0][0]);  // [kGaussTH][kGaussTW]
+#pragma unroll
+    for (int r = 0; r < static_cast<int>(kGaussTH); r++)
+      s_o[r * static_cast<int>(kGaussTW) + c] = static_cast<elem>(o[r]);
+    __syncthreads();
+    constexpr int kOutPer = static_cast<int>(kGaussTH) * CPR / static_cast<int>(kGaussTW);
+#pragma unroll
+    for (int k = 0; k < kOutPer; k++) {
+      const int q = c + k * static_cast<int>(kGaussTW);
+      const int r = q / CPR, j = q % CPR;
+      const uint32_t y = y0 + static_cast<uint32_t>(r);
+      if (y < H)
+        *reinterpret_cast<uint4*>(dst + static_cast<uint64_t>(y) * W + x0 + j * V) =
+            *reinterpret_cast<const uint4*>(s_o + r * static_cast<int>(kGaussTW) + j * V);
+    }
+    return;
+  }
   if (x >= W) return;
 #pragma unroll
   for (int r = 0; r < static_cast<int>(kGaussTH); r++) {
@@ -645,15 +735,12 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
     if (y >= H) break;
     float g = w[R] * h[r];
 #pragma unroll
-    for (int j = 1; j <= 2 * R; j++) {
-      float term = w[j < R ? R - j : j - R] * h[r + j];
-      g = g + term;
-    }
+    for (int j = 1; j <= 2 * R; j++) g = __builtin_fmaf(w[j < R ? R - j : j - R], h[r + j], g);
     float centre = s_t[r + R][c + R];
     float l = centre * p.one_minus_rate;
     float rr = g * p.rate;
     const uint32_t cell = y * W + x;
-    dst[cell] = static_cast<typename Codec<S>::elem>(encode_out<S>(l + rr, cell));
+    dst[cell] = static_cast<elem>(encode_out<S>(l + rr, cell));
   }
 }
 
@@ -661,7 +748,7 @@ template <int S>
 __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
                                                          const Instance* __restrict__ insts,
                                                          const DevParams* __restrict__ params) {
-  __shared__ float s_t[kGaussRows][kGaussCols];
+  __shared__ __align__(16) float s_t[kGaussRows][kGaussCols];
   using elem = typename Codec<S>::elem;
   const uint32_t t = blockIdx.x;
   uint32_t lo = 0, hi = a.count - 1u;  // last instance whose tiles start at or before t

@@ -28,6 +28,11 @@ CASES = [
     (512, 512, 40000, 2.0, 1.0, 1.0, "bf16sr", 10, 16, 20),
     (1000, 33, 6000, 1.0, 0.6, 0.9, "f32", 0, 5, 7),
     (64, 300, 4000, 5.0, 2.2, 1.0, "bf16", 2, 3, 8),
+    # whole 256-column tiles (16-byte chunk loads and stores), short and tall maps
+    (1024, 5, 3000, 2.0, 1.0, 1.0, "f32", 0, 4, 5),
+    (2048, 40, 30000, 3.0, 1.3, 0.7, "f16", 5, 0, 6),
+    (4096, 20, 20000, 8.0, 3.5, 1.0, "bf16sr", 0, 0, 4),
+    (768, 700, 50000, 2.0, 1.0, 1.0, "bf16", 10, 8, 12),
 ]
 
 
