@@ -116,3 +116,68 @@ def test_random_batch(slm, orc, case):
         orc.step(a, t, sp.params, steps, storage=storage)
         assert_bitwise(got[k][0], a, f"batch {case} instance {k} agents")
         assert_bitwise(got[k][1], t, f"batch {case} instance {k} trail")
+
+
+def gaussian_settings(slm, rng):
+    s = random_settings(slm, rng)
+    return dataclasses.replace(s, blur_radius=float(rng.integers(1, 9)) + float(rng.uniform(-0.45, 0.45)),
+                               blur_sigma=float(rng.uniform(0.3, 5.0)))
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_random_gaussian_single(slm, orc, case):
+    """The Gaussian diffuse extension under random shapes, parameters, storage
+    and tunings: equal to the oracle's restatement bit for bit."""
+    rng = np.random.default_rng(3000 + case)
+    w, h = side(rng), side(rng)
+    if rng.random() < 0.3:  # whole 16-byte-chunk tiles
+        w = int(rng.choice([256, 512, 768, 1024]))
+    n = int(rng.integers(0, min(2 * w * h, 30000) + 1))
+    s = gaussian_settings(slm, rng)
+    storage = STORAGES[rng.integers(len(STORAGES))]
+    tun = random_tuning(rng, batch=False)
+    chunk = int(rng.choice([0, 1, 5, 16]))
+    steps = int(rng.integers(1, 9))
+    seed = int(rng.integers(0, 2**31))
+    trail0 = rng.uniform(0.0, 1.0, (h, w)).astype(np.float32) if rng.random() < 0.5 else None
+    print(f"[random gaussian {case}] {w}x{h} n={n} {storage} steps={steps} chunk={chunk} {tun} "
+          f"r={s.blur_radius:.3f} sigma={s.blur_sigma:.3f}", flush=True)
+    with slm.Simulation(w, h, n, seed=seed, params=s, graph_chunk=chunk, gaussian=True,
+                        **storage_kw(storage)) as sim:
+        sim.set_tuning(**tun)
+        if trail0 is not None:
+            sim.write_trail(trail0)
+        a, t = sim.read_agents(), sim.read_trail()
+        sim.step(steps)
+        ga, gt = sim.read_agents(), sim.read_trail()
+    orc.step(a, t, s, steps, storage=storage, gaussian=True)
+    assert_bitwise(ga, a, f"case {case} agents")
+    assert_bitwise(gt, t, f"case {case} trail")
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_random_gaussian_batch(slm, orc, case):
+    rng = np.random.default_rng(4000 + case)
+    count = int(rng.integers(2, 9))
+    specs = []
+    for k in range(count):
+        w, h = side(rng), side(rng)
+        n = int(rng.integers(0, min(2 * w * h, 12000) + 1))
+        specs.append(slm.InstanceSpec(w, h, n, seed=int(rng.integers(0, 2**31)),
+                                      params=gaussian_settings(slm, rng)))
+    storage = STORAGES[rng.integers(len(STORAGES))]
+    tun = random_tuning(rng, batch=True)
+    chunk = int(rng.choice([0, 3, 10]))
+    steps = int(rng.integers(1, 9))
+    print(f"[random gaussian batch {case}] {[(sp.width, sp.height, sp.agent_count) for sp in specs]} "
+          f"{storage} steps={steps} chunk={chunk} {tun}", flush=True)
+    with slm.Batch(specs, graph_chunk=chunk, gaussian=True, **storage_kw(storage)) as b:
+        b.set_tuning(**tun)
+        starts = [(b.read_agents(k), b.read_trail(k)) for k in range(count)]
+        b.step(steps)
+        got = [(b.read_agents(k), b.read_trail(k)) for k in range(count)]
+    for k, sp in enumerate(specs):
+        a, t = starts[k]
+        orc.step(a, t, sp.params, steps, storage=storage, gaussian=True)
+        assert_bitwise(got[k][0], a, f"batch {case} instance {k} agents")
+        assert_bitwise(got[k][1], t, f"batch {case} instance {k} trail")
