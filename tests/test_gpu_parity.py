@@ -215,3 +215,30 @@ def test_stochastic_rounding_needs_bf16(slm):
         with pytest.raises(slm.SlimeMoldError) as e:
             slm.Simulation(16, 16, 10, storage=storage, stochastic_rounding=True)
         assert e.value.code == -5
+
+
+@pytest.mark.parametrize("resort", [0, 3])
+def test_every_agent_moves_exactly_once_above_the_reference_dispatch_limit(slm, resort):
+    """2^24 + 3 agents, far above the 4,194,240 the reference's 2-D dispatch
+    covers once per step (compute.wgsl:58-63).  Sensors placed off the grid all
+    read 0, so nobody turns, and every speed is clamped to one value: after K
+    steps each agent has moved exactly K * speed * 0.016 cells along its
+    heading, toroidally.  An agent updated twice or skipped in a step is off
+    by a whole move (0.8 cells)."""
+    import dataclasses
+    n, W, H, K = (1 << 24) + 3, 4096, 4096, 8
+    s = dataclasses.replace(preset(slm, "Default"), agent_sensor_distance=100000.0,
+                            agent_speed_min=50.0, agent_speed_max=50.0, agent_jitter=0.0)
+    print(f"[exactly once] n={n} resort={resort}", flush=True)
+    with slm.Simulation(W, H, n, seed=9, params=s, resort_interval=resort) as sim:
+        a0 = sim.read_agents()
+        sim.step(K)
+        a1 = sim.read_agents()
+    assert np.all(a1[:, 3] == np.float32(50.0))
+    assert np.array_equal(a1[:, 2], a0[:, 2])  # headings in [0, 2pi) are left as they are
+    step = K * float(np.float32(50.0) * np.float32(0.016))
+    th = a0[:, 2].astype(np.float64)
+    for axis, size, trig in ((0, W, np.cos), (1, H, np.sin)):
+        want = a0[:, axis].astype(np.float64) + step * trig(th)
+        d = np.abs(((a1[:, axis] - want) + size / 2) % size - size / 2)
+        assert d.max() < 0.02, (axis, float(d.max()), int(np.argmax(d)))
