@@ -386,8 +386,8 @@ void orc_trail_pass(const float* tin, const uint8_t* mark, uint32_t w, uint32_t 
 }
 
 /* ---- Gaussian diffuse (the library's SLM_FLAG_GAUSSIAN_DIFFUSE extension) ----
- * Not in the reference: it carries blur_radius and blur_sigma (src/main.rs:44-45,
- * src/settings.rs:24-25) but its shader only runs the 3x3 box.  Radius
+ * Not in the reference: it carries blur_radius and blur_sigma (src/main.rs:43-44,
+ * src/settings.rs:25-27,45-46) but its shader only runs the 3x3 box.  Radius
  * R = round(blur_radius) in 1..8, sigma > 0; weights w[i] = exp(-i^2/(2 sigma^2))
  * normalised over i = -R..R in double and rounded to float.  Returns R, or 0
  * (all weights 0) when the fields do not describe such a Gaussian. */

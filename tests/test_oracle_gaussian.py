@@ -1,5 +1,5 @@
 """Pins for the oracle's Gaussian diffuse (the library's opt-in extension of
-the 3x3 box; the reference carries blur_radius / blur_sigma, src/main.rs:44-45,
+the 3x3 box; the reference carries blur_radius / blur_sigma, src/main.rs:43-44,
 but never reads them), against math that does not share its code: the
 weights' closed form, the impulse response (an outer product of the weights),
 identity at rate 0, a constant field, mass conservation, and a float64 np.roll
