@@ -377,13 +377,6 @@ __device__ inline void agent_loop(const AgentSoA& a, uint32_t begin, uint32_t en
   }
 }
 
-// Workgroups of one launch are dealt round-robin to the 8 XCDs; give XCD k a
-// contiguous range of logical blocks so neighbouring agents share its L2.
-__device__ inline uint32_t xcd_logical_block(uint32_t b, uint32_t nb) {
-  uint32_t xcd = b & 7u, idx = b >> 3;
-  uint32_t per = nb >> 3, rem = nb & 7u;
-  return xcd < rem ? xcd * (per + 1u) + idx : rem * (per + 1u) + (xcd - rem) * per + idx;
-}
 
 struct AgentArgs {
   AgentSoA a;

@@ -118,6 +118,15 @@ constexpr int kAgentBlock = 256;
 constexpr int kTrailBlock = 256;
 constexpr int kTrailWavesPerBlock = kTrailBlock / 64;
 
+// Workgroups of one launch are dealt round-robin to the 8 XCDs; this gives
+// XCD k a contiguous range of logical blocks (a bijection on [0, nb)), so
+// neighbouring blocks' data share that XCD's L2.
+__device__ inline uint32_t xcd_logical_block(uint32_t b, uint32_t nb) {
+  uint32_t xcd = b & 7u, idx = b >> 3;
+  uint32_t per = nb >> 3, rem = nb & 7u;
+  return xcd < rem ? xcd * (per + 1u) + idx : rem * (per + 1u) + (xcd - rem) * per + idx;
+}
+
 inline uint64_t storage_bytes(int storage) { return storage == SLM_STORAGE_F32 ? 4 : 2; }
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }
 

@@ -61,11 +61,15 @@ constexpr uint32_t kCountSlots = kCountPer * kAgentBlock;
 
 __global__ __launch_bounds__(kAgentBlock) void count_kernel(
     AgentSoA a, const Instance* insts, const uint32_t* block_inst, uint32_t ninst, uint32_t total,
-    int pos64, uint32_t cell_log2, uint32_t* rank, uint32_t* count) {
+    int pos64, uint32_t cell_log2, int xcd_remap, uint32_t* rank, uint32_t* count) {
   __shared__ uint32_t s_min, s_max;
   __shared__ uint32_t s_cnt[kLocalBins];
   __shared__ uint32_t s_base[kLocalBins];
-  const uint32_t base = blockIdx.x * kCountSlots;
+  // XCD-contiguous blocks, as in the scatter: neighbouring slots, their bins'
+  // counters and their output ranges stay with one XCD's L2 (the re-sort of
+  // the 256-instance mix measured 10% faster)
+  const uint32_t base =
+      (xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x) * kCountSlots;
   uint32_t k[kCountPer];
   bool act[kCountPer];
   float px[kCountPer], py[kCountPer];
@@ -216,8 +220,10 @@ __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(uint32_t* out, uin
 // read back: 8 bytes per agent less traffic, ~15% of the re-sort.
 __global__ __launch_bounds__(kAgentBlock) void scatter_kernel(
     AgentSoA src, AgentSoA dst, const Instance* insts, const uint32_t* block_inst, uint32_t ninst,
-    uint32_t n, int pos64, uint32_t cell_log2, const uint32_t* rank, const uint32_t* offset) {
-  uint32_t first = blockIdx.x * kAgentBlock;
+    uint32_t n, int pos64, uint32_t cell_log2, int xcd_remap, const uint32_t* rank,
+    const uint32_t* offset) {
+  const uint32_t blk = xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  uint32_t first = blk * kAgentBlock;
   uint32_t last = min(first + kAgentBlock, n) - 1u;
   uint32_t j = first + threadIdx.x;
   bool act = j < n;
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(kAgentBlock) void scatter_kernel(
     }
   }
   uint32_t k = 0;
-  for (uint32_t ii = block_inst[blockIdx.x];; ii++) {  // block-uniform walk
+  for (uint32_t ii = block_inst[blk];; ii++) {  // block-uniform walk
     const Instance in = insts[ii];
     uint32_t end = in.agent_off + in.agent_count;
     if (act && j >= in.agent_off && j < end) k = bin_of(x, y, in, cell_log2);
@@ -366,11 +372,11 @@ void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& d
   zero_u32_kernel<<<blocks_for(pool.total_bins, 256), 256, 0, stream>>>(s.count, pool.total_bins);
   count_kernel<<<blocks_for(pool.total_agents, kCountSlots), kAgentBlock, 0, stream>>>(
       src, pool.inst, pool.agent_block_inst, pool.count, pool.total_agents, pool.pos64,
-      pool.sort_cell_log2, s.rank, s.count);
+      pool.sort_cell_log2, pool.xcd_remap, s.rank, s.count);
   scan(s, s.count, pool.total_bins, s.offset, 0, stream);
   scatter_kernel<<<blocks_for(pool.total_agents, kAgentBlock), kAgentBlock, 0, stream>>>(
       src, dst, pool.inst, pool.agent_block_inst, pool.count, pool.total_agents, pool.pos64,
-      pool.sort_cell_log2, s.rank, s.offset);
+      pool.sort_cell_log2, pool.xcd_remap, s.rank, s.offset);
   if (pool.items) {  // work list of the tiled agent pass
     uint32_t nt = pool.total_tiles;
     tile_count_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(

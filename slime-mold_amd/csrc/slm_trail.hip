@@ -25,12 +25,6 @@
 namespace slm {
 namespace {
 
-__device__ inline uint32_t xcd_logical_block(uint32_t b, uint32_t nb) {
-  uint32_t xcd = b & 7u, idx = b >> 3;
-  uint32_t per = nb >> 3, rem = nb & 7u;
-  return xcd < rem ? xcd * (per + 1u) + idx : rem * (per + 1u) + (xcd - rem) * per + idx;
-}
-
 // ---- storage formats -------------------------------------------------------
 // Elements are handled as raw bits; the sign bit is the deposit mark.
 template <int S>
