@@ -2,7 +2,8 @@
 
 No counterpart in the reference, which runs one simulation per process
 (src/main.rs:147).  Each step is one agent launch and one trail launch over
-all instances (an instance-offset table drives both); instance k gives bit for
+all instances (an instance-offset table drives both), or with instance groups
+one such pair per group, spread over four streams; instance k gives bit for
 bit what a Simulation with the same size, agent count, seed and parameters
 gives.
 """
