@@ -564,8 +564,9 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
 // ---- Gaussian diffuse (SLM_FLAG_GAUSSIAN_DIFFUSE) ------------------------------
 // One 256-thread workgroup per tile of kGaussTW x kGaussTH cells.  The tile's
 // deposited and decayed cells plus an R-cell toroidal halo on every side are
-// staged in LDS (thread c loads column c, and column c + 256 of the halo);
-// thread c then runs the horizontal pass for column c over every row of the
+// staged in LDS (as 16-byte chunks where the tile lies inside a row of whole
+// chunks, else one cell per lane: see gauss_tile); thread c then runs the
+// horizontal pass for column c over every row of the
 // tile and its halo into registers, and the vertical pass from them.  Both
 // passes start from the product with the first tap's weight and add the others
 // in ascending tap order with one fused multiply-add each, so the result is
