@@ -143,15 +143,13 @@ __device__ inline uint32_t encode_out(float v, uint32_t cell) {
     return Codec<S>::encode(v);
 }
 
-// Deposit (S3) and decay of one stored element.
+// Deposit (S3) and decay of one stored element.  clamp(d, 0, 1) is one
+// v_med3_f32: d = |T| + deposit is finite and never -0 (|T| >= +0), so the
+// median of (d, 0, 1) is exactly the clamp.
 template <int S>
 __device__ inline float deposit_decay(uint32_t raw, const DevParams& p) {
   float t = Codec<S>::value(raw);
-  if (Codec<S>::marked(raw)) {
-    float d = t + p.deposit;
-    d = d > 0.0f ? d : 0.0f;
-    t = d < 1.0f ? d : 1.0f;
-  }
+  if (Codec<S>::marked(raw)) t = __builtin_amdgcn_fmed3f(t + p.deposit, 0.0f, 1.0f);
   float u = t - p.decay_k;
   return u > 0.0f ? u : 0.0f;
 }
