@@ -242,3 +242,24 @@ def test_every_agent_moves_exactly_once_above_the_reference_dispatch_limit(slm, 
         want = a0[:, axis].astype(np.float64) + step * trig(th)
         d = np.abs(((a1[:, axis] - want) + size / 2) % size - size / 2)
         assert d.max() < 0.02, (axis, float(d.max()), int(np.argmax(d)))
+
+
+def test_c1_headline_shape_16_steps(slm, orc):
+    """The plain leg's own configuration, bit for bit against the oracle for the
+    first 16 steps: 4096x4096 f32, 2^24 agents, seed 1, presets Default ->
+    Sponge -> Firecracker Trees switched between graph chunks, re-sort every
+    15 steps into 8x8 bins, streaming trail pass."""
+    names = ["Default", "Sponge", "Firecracker Trees"]
+    print("[c1 parity] 4096x4096 n=2^24 16 steps", flush=True)
+    with slm.Simulation(4096, 4096, 1 << 24, seed=1, params=preset(slm, names[0]),
+                        resort_interval=15, graph_chunk=4, sort_cell_log2=3) as sim:
+        sim.set_tuning(trail_rows=16, streaming=3)
+        a, t = sim.read_agents(), sim.read_trail()
+        for name, n in zip(names, (4, 4, 8)):
+            sim.set_params(preset(slm, name))
+            sim.step(n)
+        ga, gt = sim.read_agents(), sim.read_trail()
+    for name, n in zip(names, (4, 4, 8)):
+        orc.step(a, t, preset(slm, name), n)
+    assert_bitwise(ga, a, "agents")
+    assert_bitwise(gt, t, "trail")
