@@ -215,3 +215,28 @@ def test_instance_groups_with_params_clear_render_and_mass(slm):
         assert_bitwise(i5[k], i1[k], f"instance {k} render")
         assert_bitwise(s5[k][0], s1[k][0], f"instance {k} agents")
         assert_bitwise(s5[k][1], s1[k][1], f"instance {k} trail")
+
+
+def test_c3_mix_sampled_instances_16_steps(slm, orc):
+    """The batched leg's own workload: the 256-instance mix in bf16 with
+    stochastic rounding, 128 instance groups, re-sort every 20 steps, 16 steps
+    in graph chunks.  Instances are independent, so a sample of them (every
+    grid size, small and large agent counts) is checked against the oracle bit
+    for bit."""
+    import slimemold.workloads as wl
+    specs = wl.batch_mix(256)
+    sample = sorted({0, 1, 2, 3, 64, 127, 128, 200, 255} |
+                    {min(range(256), key=lambda k: specs[k].agent_count),
+                     max(range(256), key=lambda k: specs[k].agent_count)})
+    print(f"[c3 parity] 256 instances, sampled {sample}", flush=True)
+    with slm.Batch(specs, storage="bf16", stochastic_rounding=True, resort_interval=20,
+                   graph_chunk=4, sort_cell_log2=3) as b:
+        b.set_tuning(trail_rows=8, instance_groups=128)
+        starts = {k: (b.read_agents(k), b.read_trail(k)) for k in sample}
+        b.step(16)
+        got = {k: (b.read_agents(k), b.read_trail(k)) for k in sample}
+    for k in sample:
+        a, t = starts[k]
+        orc.step(a, t, specs[k].params, 16, storage="bf16sr")
+        assert_bitwise(got[k][0], a, f"instance {k} agents")
+        assert_bitwise(got[k][1], t, f"instance {k} trail")
