@@ -240,3 +240,27 @@ def test_c3_mix_sampled_instances_16_steps(slm, orc):
         orc.step(a, t, specs[k].params, 16, storage="bf16sr")
         assert_bitwise(got[k][0], a, f"instance {k} agents")
         assert_bitwise(got[k][1], t, f"instance {k} trail")
+
+
+def test_c4_big_instance_two_steps(slm, orc):
+    """The combined workload (1024 mixed instances plus one 16384^2 instance
+    with 2^28 agents, bf16 with stochastic rounding, 512 instance groups):
+    after 30 steps, two more steps of the big instance and two small ones
+    equal the oracle's bit for bit."""
+    import slimemold.workloads as wl
+    specs = wl.combined()
+    big = len(specs) - 1
+    sample = [0, 511, big]
+    print(f"[c4 parity] {len(specs)} instances, big {specs[big].width}^2", flush=True)
+    with slm.Batch(specs, storage="bf16", stochastic_rounding=True, resort_interval=20,
+                   graph_chunk=4, sort_cell_log2=3) as b:
+        b.set_tuning(trail_rows=8, instance_groups=512)
+        b.step(30)
+        starts = {k: (b.read_agents(k), b.read_trail(k)) for k in sample}
+        b.step(2)
+        got = {k: (b.read_agents(k), b.read_trail(k)) for k in sample}
+    for k in sample:
+        a, t = starts[k]
+        orc.step(a, t, specs[k].params, 2, storage="bf16sr")
+        assert_bitwise(got[k][0], a, f"instance {k} agents")
+        assert_bitwise(got[k][1], t, f"instance {k} trail")
