@@ -264,3 +264,23 @@ def test_c4_big_instance_two_steps(slm, orc):
         orc.step(a, t, specs[k].params, 2, storage="bf16sr")
         assert_bitwise(got[k][0], a, f"instance {k} agents")
         assert_bitwise(got[k][1], t, f"instance {k} trail")
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_c2_shape_with_f64_positions_4_steps(slm, orc, storage):
+    """The precision configuration itself: 8192^2 trail in bf16 or f16, 2^26
+    agents, seed 7, Default preset, f64 positions (the mode meant for grids of
+    8192 and larger), re-sorting on; 4 steps bit for bit against the oracle's
+    f64-position restatement."""
+    s = preset(slm, "Default")
+    print(f"[c2 parity] 8192^2 n=2^26 {storage} f64 positions", flush=True)
+    with slm.Simulation(8192, 8192, 1 << 26, seed=7, params=s, storage=storage,
+                        f64_positions=True, resort_interval=2, graph_chunk=2) as sim:
+        a0 = sim.read_agents()
+        t = sim.read_trail()
+        sim.step(4)
+        ga, gt = sim.read_agents(), sim.read_trail()
+    a = a0.astype(np.float64)
+    orc.step_p64(a, t, s, 4, storage=storage)
+    assert_bitwise(ga, a.astype(np.float32), "f64-position agents (rounded to f32)")
+    assert_bitwise(gt, t, "trail")
