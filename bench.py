@@ -122,6 +122,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-single-thread", type=int, default=1,
+                    help="also time one whole step of the CPU baseline on one thread (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB device to device three times "
                          "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes)")
@@ -532,6 +534,16 @@ def main():
             "sample": (f"C oracle (OpenMP, {threads} threads) on the plain leg's {W}x{H}, "
                        f"{N}-agent initial state, Default preset, {n} whole steps in {dt:.1f} s"),
         }
+        if args.cpu_single_thread:
+            a1, t1 = a0.copy(), t0.copy()
+            t_start = time.perf_counter()
+            oracle.step(a1, t1, plain["presets"][0], 1, nthreads=1)
+            dt1 = time.perf_counter() - t_start
+            out["cpu_baseline_1thread"] = {
+                "value": N / dt1, "unit": "agent-updates/s", "cores": 1, "kind": "port",
+                "sample": (f"C oracle on 1 thread, the same state and preset, 1 whole step "
+                           f"in {dt1:.1f} s"),
+            }
     print(json.dumps(out))
     return 0
 
