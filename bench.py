@@ -149,6 +149,20 @@ def copy_probe(mib):
     return 0
 
 
+def cpu_model():
+    """The host CPU's model name and the CPUs this process may run on."""
+    name = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name}, {len(os.sched_getaffinity(0))} CPUs available"
+
+
 def cpu_baseline(orc, agents, trail, settings, seconds, threads):
     """Oracle (C, OpenMP) on the same state: one step to size the sample, then
     as many whole steps as fit in `seconds`."""
@@ -533,6 +547,7 @@ def main():
             "kind": "port",
             "sample": (f"C oracle (OpenMP, {threads} threads) on the plain leg's {W}x{H}, "
                        f"{N}-agent initial state, Default preset, {n} whole steps in {dt:.1f} s"),
+            "host": cpu_model(),
         }
         if args.cpu_single_thread:
             a1, t1 = a0.copy(), t0.copy()
