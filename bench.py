@@ -502,6 +502,7 @@ def main():
                           f"{plain['measure_steps']} steps (same presets, thirds) right after the "
                           f"timed region; those steps ran at "
                           f"{plain['measure_value']:.4g} agent-updates/s"),
+        "cell_updates_per_s": plain["cells"] * plain["steps"] / plain["elapsed"],
         "final_total_mass": plain["mass"],
         "cpu_baseline": None,
     }
