@@ -2216,6 +2216,9 @@ int slm_batch_create(const slm_batch_config* cfg, const slm_instance_desc* insts
     int rc = check_grid(d.width, d.height);
     if (!rc) rc = check_params(&d.params, cfg->flags);
     if (rc) return fail(rc, "instance %u: %s", k, g_last_error);
+    // bounded before the add, so the running total cannot wrap
+    if (d.agent_count > kMaxAgents || agents + d.agent_count > kMaxAgents)
+      return fail(SLM_ERR_INVALID_ARGUMENT, "instance %u: total agents must be < 2^32 - 256", k);
     agents += d.agent_count;
     cells += static_cast<uint64_t>(d.width) * d.height;
     specs.push_back({d.width, d.height, d.agent_count, d.seed, d.params});

@@ -121,6 +121,24 @@ def test_create_rejects_bad_params(slm):
     assert slm.lib().slm_create(ctypes.byref(cfg), ctypes.byref(p), None) == -1
 
 
+@pytest.mark.parametrize("counts", [
+    [(1 << 64) - 1, 1],          # the sum wraps to 0 in 64 bits
+    [(1 << 63), (1 << 63)],      # so does this one
+    [1 << 40],                   # one instance beyond the slot range
+    [0xffffff00, 1],             # the total just beyond it
+])
+def test_batch_create_rejects_agent_totals_before_any_launch(slm, counts):
+    descs = (slm._native.InstanceDesc * len(counts))()
+    for d, n in zip(descs, counts):
+        d.width, d.height, d.agent_count, d.seed = 16, 16, n, 1
+        d.params = slm.default_params()
+    cfg = slm.default_batch_config()
+    h = ctypes.c_void_p()
+    rc = slm.lib().slm_batch_create(ctypes.byref(cfg), descs, len(counts), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert "total agents" in slm.lib().slm_last_error().decode()
+
+
 def test_null_handle_calls_fail_cleanly(slm):
     L = slm.lib()
     assert L.slm_step(None, 1) == -1
