@@ -356,7 +356,9 @@ int slm_batch_autotune(slm_batch* batch, uint32_t trial_steps, slm_tuning* out);
  * src/main.rs:1199-1215) and the LUT buffer it reads (src/main.rs:330-342).
  * lut: 768 bytes, R[256] then G[256] then B[256], the layout of the reference's
  * .lut files and LutData (src/lut_manager.rs:4-18,162-186).  Renders the
- * current trail into a tex_w x tex_h RGBA8 image, row-major, 4 bytes per pixel
+ * current trail (as it stands after the last step, i.e. after the diffuse;
+ * slm_step_render_rgba8 below renders at the reference's own sampling point)
+ * into a tex_w x tex_h RGBA8 image, row-major, 4 bytes per pixel
  * (r, g, b, 255): the grid is fitted into the texture keeping its aspect ratio
  * and centred (f32 scale and offsets as the shader computes them), each pixel
  * takes its cell by truncation, the intensity clamp(T, 0, 1) selects table
@@ -370,6 +372,22 @@ int slm_render_rgba8_device(slm_sim* sim, const uint8_t* lut, uint32_t tex_w, ui
                             void* rgba_device);
 int slm_batch_render_rgba8(slm_batch* batch, uint32_t index, const uint8_t* lut, uint32_t tex_w,
                            uint32_t tex_h, uint8_t* rgba);
+/* The calls above show the trail after the last step's diffuse.  The
+ * reference's frame displays it earlier: agents, decay, display, diffuse
+ * (src/main.rs:1163-1235; the display pass at :1199-1215 samples the trail
+ * between the decay and diffuse submits).  slm_step_render_rgba8 enqueues
+ * n_steps >= 1 steps, as slm_step would, and renders the last one at that
+ * point: each pixel's cell shows clamp(T + deposit, 0, 1) where an agent of
+ * that step landed, else T, minus the decay and floored at 0 - the trail
+ * after deposit and decay, before the diffuse (rendered in f32, not rounded
+ * to the storage format).  The image is the one the reference would show
+ * for that frame; the state afterwards equals slm_step(n_steps)'s.  The last
+ * step is launched eagerly (not from a captured graph).  The host form waits
+ * for the image; the device form only enqueues. */
+int slm_step_render_rgba8(slm_sim* sim, uint32_t n_steps, const uint8_t* lut, uint32_t tex_w,
+                          uint32_t tex_h, uint8_t* rgba);
+int slm_step_render_rgba8_device(slm_sim* sim, uint32_t n_steps, const uint8_t* lut,
+                                 uint32_t tex_w, uint32_t tex_h, void* rgba_device);
 
 /* ---- snapshots ------------------------------------------------------------------
  * A resumable copy of a single simulation.  Loading it into a simulation of

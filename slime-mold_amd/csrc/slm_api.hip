@@ -531,30 +531,39 @@ class Engine {
   }
 
   // ---- display colour map -------------------------------------------------------
+  // Renders instance k's current trail (after its last step's diffuse) into
+  // `out`: device memory, or host memory through a short-lived staging buffer.
   int render(size_t k, const uint8_t* lut768, uint32_t tex_w, uint32_t tex_h, void* out,
              bool device_out) {
-    LutBytes lut;
-    std::memcpy(lut.word, lut768, sizeof lut.word);
-    RenderArgs a{};
-    render_fit(spec[k].width, spec[k].height, tex_w, tex_h, a);
+    return render_at(k, lut768, tex_w, tex_h, out, device_out, nullptr, 0);
+  }
+
+  // As render, but into host memory through `staging`, a device buffer of at
+  // least tex_w * tex_h * 4 bytes the caller owns (nothing is allocated).
+  int render_via(size_t k, const uint8_t* lut768, uint32_t tex_w, uint32_t tex_h, void* out,
+                 void* staging) {
+    RenderArgs a = render_args(k, tex_w, tex_h);
     a.trail = trail_ptr(cur_, k);
-    if (device_out) {
-      a.out = static_cast<uint32_t*>(out);
-      launch_render(a, lut, cfg.storage, stream);
-      SLM_HIP(hipGetLastError());
-      return SLM_OK;
-    }
-    uint64_t bytes = static_cast<uint64_t>(tex_w) * tex_h * 4u;
-    void* staging = nullptr;
-    SLM_TRY(scratch_alloc(&staging, bytes));
     a.out = static_cast<uint32_t*>(staging);
-    launch_render(a, lut, cfg.storage, stream);
+    launch_render(a, lut_of(lut768), cfg.storage, stream);
+    uint64_t bytes = static_cast<uint64_t>(tex_w) * tex_h * 4u;
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, staging, bytes, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e == hipSuccess) e = hipMemcpy(out, staging, bytes, hipMemcpyDeviceToHost);
-    (void)hipFree(staging);
     if (e != hipSuccess) return fail(SLM_ERR_HIP, "render failed: %s", hipGetErrorString(e));
     return SLM_OK;
+  }
+
+  // Enqueues n >= 1 steps (pool of one); the last runs eagerly with the
+  // display between its agent pass and its trail pass, as the reference's
+  // frame does (agents, decay, display, diffuse: src/main.rs:1163-1235): the
+  // image shows the trail after the agents' deposits and the decay, before
+  // the diffuse.  The render kernel reads the buffer the agent pass has just
+  // marked and applies each sampled cell's deposit and decay itself.
+  int step_render(uint32_t n, const uint8_t* lut768, uint32_t tex_w, uint32_t tex_h, void* out,
+                  bool device_out) {
+    SLM_TRY(step(n - 1));
+    return render_at(0, lut768, tex_w, tex_h, out, device_out, params_d_, 1);
   }
 
   // ---- snapshots (single simulation) -------------------------------------------
@@ -730,6 +739,49 @@ class Engine {
   }
 
  private:
+  static LutBytes lut_of(const uint8_t* lut768) {
+    LutBytes lut;
+    std::memcpy(lut.word, lut768, sizeof lut.word);
+    return lut;
+  }
+
+  RenderArgs render_args(size_t k, uint32_t tex_w, uint32_t tex_h) const {
+    RenderArgs a{};
+    render_fit(spec[k].width, spec[k].height, tex_w, tex_h, a);
+    return a;
+  }
+
+  // steps: 0 renders the current trail; 1 enqueues one step with the render
+  // between its passes, decoding deposit and decay with `decode`.
+  int render_at(size_t k, const uint8_t* lut768, uint32_t tex_w, uint32_t tex_h, void* out,
+                bool device_out, const DevParams* decode, uint32_t steps) {
+    RenderArgs a = render_args(k, tex_w, tex_h);
+    const LutBytes lut = lut_of(lut768);
+    uint64_t bytes = static_cast<uint64_t>(tex_w) * tex_h * 4u;
+    void* staging = nullptr;
+    if (!device_out) SLM_TRY(scratch_alloc(&staging, bytes));
+    a.out = static_cast<uint32_t*>(device_out ? out : staging);
+    a.decode = decode;
+    auto draw = [&] {
+      a.trail = trail_ptr(cur_, k);
+      launch_render(a, lut, cfg.storage, stream);
+    };
+    if (steps) {
+      enqueue_step(sort_due(steps_done), draw);
+      steps_done++;
+    } else {
+      draw();
+    }
+    hipError_t e = hipGetLastError();
+    if (!device_out) {
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e == hipSuccess) e = hipMemcpy(out, staging, bytes, hipMemcpyDeviceToHost);
+      (void)hipFree(staging);
+    }
+    if (e != hipSuccess) return fail(SLM_ERR_HIP, "render failed: %s", hipGetErrorString(e));
+    return SLM_OK;
+  }
+
   // Agents in original order (and optionally the current trail), in
   // short-lived buffers.
   struct Snapshot {
@@ -1321,7 +1373,10 @@ class Engine {
       launch_agents(p, agents_[acur_], trail_[cur_], stream);
   }
 
-  void enqueue_step(bool do_sort) {
+  // One step; `between` runs after the agent pass is enqueued and before the
+  // trail pass (the reference's display point).
+  template <class Between>
+  void enqueue_step(bool do_sort, Between&& between) {
     Pool p = pool();
     if (do_sort && sorting() && total_agents_ > 0) {
       launch_morton_sort(p, agents_[acur_], agents_[acur_ ^ 1], sort_, stream);
@@ -1329,8 +1384,12 @@ class Engine {
       tiles_ready_ = true;
     }
     enqueue_agents(p);
+    between();
     launch_trail(p, trail_[cur_], trail_[cur_ ^ 1], stream);
     cur_ ^= 1;
+  }
+  void enqueue_step(bool do_sort) {
+    enqueue_step(do_sort, [] {});
   }
 
   // ---- instance groups ---------------------------------------------------------
@@ -2114,6 +2173,26 @@ int slm_render_rgba8_device(slm_sim* s, const uint8_t* lut, uint32_t tex_w, uint
   SIM_GUARD(s);
   SLM_TRY(check_render(lut, tex_w, tex_h, rgba_device));
   return s->eng->render(0, lut, tex_w, tex_h, rgba_device, true);
+  SLM_ENTRY_END
+}
+
+int slm_step_render_rgba8(slm_sim* s, uint32_t n_steps, const uint8_t* lut, uint32_t tex_w,
+                          uint32_t tex_h, uint8_t* rgba) {
+  SLM_ENTRY_BEGIN
+  SIM_GUARD(s);
+  SLM_TRY(check_render(lut, tex_w, tex_h, rgba));
+  if (n_steps == 0) return fail(SLM_ERR_INVALID_ARGUMENT, "n_steps must be >= 1");
+  return s->eng->step_render(n_steps, lut, tex_w, tex_h, rgba, false);
+  SLM_ENTRY_END
+}
+
+int slm_step_render_rgba8_device(slm_sim* s, uint32_t n_steps, const uint8_t* lut,
+                                 uint32_t tex_w, uint32_t tex_h, void* rgba_device) {
+  SLM_ENTRY_BEGIN
+  SIM_GUARD(s);
+  SLM_TRY(check_render(lut, tex_w, tex_h, rgba_device));
+  if (n_steps == 0) return fail(SLM_ERR_INVALID_ARGUMENT, "n_steps must be >= 1");
+  return s->eng->step_render(n_steps, lut, tex_w, tex_h, rgba_device, true);
   SLM_ENTRY_END
 }
 

@@ -176,6 +176,12 @@ struct LutBytes {
 };
 struct RenderArgs {
   const void* trail;  // one instance's trail
+  // nullptr: the stored values (after the diffuse).  Else the instance's
+  // parameters, and each sampled cell shows its deposit and decay applied to
+  // the raw element - the trail as the reference displays it, between its
+  // decay and diffuse passes (src/main.rs:1183-1217) - when `trail` is the
+  // buffer the agent pass has just marked.
+  const DevParams* decode;
   uint32_t width;     // cells per trail row
   float sim_w, sim_h;
   uint32_t tex_w, tex_h;

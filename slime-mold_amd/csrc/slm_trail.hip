@@ -482,8 +482,9 @@ __global__ void encode_trail_kernel(const float* in, uint64_t cells, void* t) {
 // the 768-byte R[256] G[256] B[256] table, and stores RGBA bytes (alpha 255).
 // The shader stores f32(byte) / 255 to an rgba8unorm texture, which converts
 // back to the same byte, so the bytes are the table's.  Pixels outside the
-// fitted grid are black.
-template <int S>
+// fitted grid are black.  kDecode: the cell's value is deposit_decay of its
+// raw element (the marked buffer between the agent and trail passes).
+template <int S, bool kDecode>
 __global__ __launch_bounds__(256) void render_kernel(RenderArgs a, LutBytes lut) {
   __shared__ uint32_t s_lut[192];
   if (threadIdx.x < 192) s_lut[threadIdx.x] = lut.word[threadIdx.x];
@@ -497,7 +498,8 @@ __global__ __launch_bounds__(256) void render_kernel(RenderArgs a, LutBytes lut)
   if (fx >= 0.0f && fx < a.sim_w && fy >= 0.0f && fy < a.sim_h) {
     uint64_t idx = static_cast<uint64_t>(static_cast<int>(fy)) * a.width +
                    static_cast<uint64_t>(static_cast<int>(fx));
-    float t = Codec<S>::value(static_cast<const typename Codec<S>::elem*>(a.trail)[idx]);
+    uint32_t raw = static_cast<const typename Codec<S>::elem*>(a.trail)[idx];
+    float t = kDecode ? deposit_decay<S>(raw, *a.decode) : Codec<S>::value(raw);
     t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
     uint32_t k = static_cast<uint32_t>(t * 255.0f);
     auto byte = [&](uint32_t b) { return (s_lut[b >> 2] >> ((b & 3u) * 8u)) & 0xffu; };
@@ -927,12 +929,16 @@ void launch_render(const RenderArgs& a, const LutBytes& lut, int storage, hipStr
   uint64_t px = static_cast<uint64_t>(a.tex_w) * a.tex_h;
   if (px == 0) return;
   uint32_t nb = blocks_for(px, 256);
+  const bool dec = a.decode != nullptr;
   if (storage == SLM_STORAGE_BF16)
-    render_kernel<SLM_STORAGE_BF16><<<nb, 256, 0, stream>>>(a, lut);
+    dec ? render_kernel<SLM_STORAGE_BF16, true><<<nb, 256, 0, stream>>>(a, lut)
+        : render_kernel<SLM_STORAGE_BF16, false><<<nb, 256, 0, stream>>>(a, lut);
   else if (storage == SLM_STORAGE_F16)
-    render_kernel<SLM_STORAGE_F16><<<nb, 256, 0, stream>>>(a, lut);
+    dec ? render_kernel<SLM_STORAGE_F16, true><<<nb, 256, 0, stream>>>(a, lut)
+        : render_kernel<SLM_STORAGE_F16, false><<<nb, 256, 0, stream>>>(a, lut);
   else
-    render_kernel<SLM_STORAGE_F32><<<nb, 256, 0, stream>>>(a, lut);
+    dec ? render_kernel<SLM_STORAGE_F32, true><<<nb, 256, 0, stream>>>(a, lut)
+        : render_kernel<SLM_STORAGE_F32, false><<<nb, 256, 0, stream>>>(a, lut);
 }
 
 void launch_total_mass(const Pool& pool, const void* t, double* partials, double* out,

@@ -202,6 +202,10 @@ SIGNATURES = {
                                 ctypes.c_void_p],
     "slm_batch_render_rgba8": [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
                                ctypes.c_uint32, ctypes.c_void_p],
+    "slm_step_render_rgba8": [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                              ctypes.c_uint32, ctypes.c_void_p],
+    "slm_step_render_rgba8_device": [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_void_p],
     "slm_snapshot_size": [_P, ctypes.POINTER(ctypes.c_uint64)],
     "slm_save_snapshot": [_P, ctypes.c_void_p, ctypes.c_uint64],
     "slm_load_snapshot": [_P, ctypes.c_void_p, ctypes.c_uint64],

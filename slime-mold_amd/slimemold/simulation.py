@@ -160,6 +160,16 @@ class Simulation:
         N.check(N.lib().slm_render_rgba8(self.handle, lut, tex_w, tex_h, out.ctypes.data))
         return out
 
+    def step_render(self, n_steps: int, lut, tex_w: int, tex_h: int) -> np.ndarray:
+        """Advance n_steps (>= 1) steps and return the last one's image as the
+        reference's frame shows it: after the agents' deposits and the decay,
+        before the diffuse (src/main.rs:1163-1235)."""
+        lut = _lut_bytes(lut)
+        out = np.empty((tex_h, tex_w, 4), np.uint8)
+        N.check(N.lib().slm_step_render_rgba8(self.handle, n_steps, lut, tex_w, tex_h,
+                                              out.ctypes.data))
+        return out
+
     def snapshot(self, path: Optional[str] = None) -> bytes:
         """The simulation's resumable state (written to `path` if given)."""
         n = ctypes.c_uint64()

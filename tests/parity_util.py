@@ -46,3 +46,19 @@ def run_parity(slm, orc, w, h, n, seed, settings, steps, storage="f32", trail0=N
         ga, gt = sim.read_agents(), sim.read_trail()
     orc.step(a, t, settings, steps, storage=storage)
     return ga, gt, a, t
+
+
+def display_point(orc, agents, trail, settings, storage="f32", gaussian=False):
+    """One oracle step from (agents, trail), in place, returning the trail as
+    the reference's frame displays it: after the agents' deposits and the
+    decay, before the diffuse (src/main.rs:1183-1217).  That is the trail pass
+    with diffusion rate 0 (mix(T2, avg, 0) = T2) over the cells the moved
+    agents deposit into, kept in f32 (the display samples the decayed values
+    before any storage rounding)."""
+    before = trail.copy()
+    orc.step(agents, trail, settings, 1, storage=storage, gaussian=gaussian)
+    h, w = trail.shape
+    op = orc.params(settings)
+    d = {n: getattr(op, n) for n, _ in op._fields_}
+    d["diffusion_rate"] = 0.0
+    return orc.trail_pass(before, d, mark=orc.deposit_marks(agents, w, h), storage="f32")

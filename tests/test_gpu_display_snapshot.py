@@ -2,7 +2,9 @@
 
 Render (src/display.wgsl:29-87): the HIP pass equals the oracle's restatement
 byte for byte, for grids and textures of every aspect and storage format, into
-host or device memory.  Snapshots: a run saved and resumed (in the same
+host or device memory; rendered at the reference's own sampling point (after
+the deposits and the decay, before the diffuse: src/main.rs:1183-1217) it
+equals the oracle's render of that intermediate trail.  Snapshots: a run saved and resumed (in the same
 simulation, a fresh one, or one created from the snapshot) equals the run that
 was never interrupted, bit for bit; a snapshot built by an independent encoder
 of the documented layout loads as the state it describes.
@@ -13,7 +15,7 @@ import numpy as np
 import pytest
 
 import snapshot_util as su
-from parity_util import assert_bitwise, preset
+from parity_util import assert_bitwise, display_point, preset
 
 pytestmark = pytest.mark.gpu
 
@@ -73,6 +75,51 @@ def test_render_into_device_memory_and_batches(slm, orc):
             lut2 = random_lut(10 + k)
             assert_bitwise(b.render(k, lut2, 333, 222),
                            orc.render(b.read_trail(k), lut2, 333, 222), f"batch render {k}")
+
+
+STEP_RENDER_CASES = [
+    dict(storage="f32", name="Default", resort=0, gaussian=False, shape=(1600, 900, 30000)),
+    dict(storage="bf16", name="Firecracker Trees", resort=5, gaussian=False,
+         shape=(257, 1031, 6000)),
+    dict(storage="f16", name="Mesh", resort=3, gaussian=False, shape=(300, 200, 9000)),
+    dict(storage="f32", name="Snake", resort=4, gaussian=True, shape=(128, 96, 3000)),
+]
+
+
+@pytest.mark.parametrize("case", STEP_RENDER_CASES,
+                         ids=lambda c: f"{c['storage']}-{c['name'].replace(' ', '')}")
+def test_step_render_shows_the_reference_sampling_point(slm, orc, case):
+    w, h, n = case["shape"]
+    s = preset(slm, case["name"])
+    lut = random_lut(21)
+    kw = dict(seed=6, params=s, storage=case["storage"], resort_interval=case["resort"],
+              graph_chunk=4, gaussian=case["gaussian"])
+    with slm.Simulation(w, h, n, **kw) as sim:
+        sim.step(7)
+        a, t = sim.read_agents(), sim.read_trail()
+        for k, (steps, tw, th) in enumerate([(1, w, h), (6, 1920, 1080), (3, 7, 3)]):
+            print(f"[step_render] {case} steps={steps} tex={tw}x{th}", flush=True)
+            got = sim.step_render(steps, lut, tw, th)
+            orc.step(a, t, s, steps - 1, storage=case["storage"], gaussian=case["gaussian"])
+            shown = display_point(orc, a, t, s, case["storage"], gaussian=case["gaussian"])
+            assert_bitwise(got, orc.render(shown, lut, tw, th), f"step_render {k}")
+            # the state afterwards is slm_step's
+            assert_bitwise(sim.read_agents(), a, f"agents after step_render {k}")
+            assert_bitwise(sim.read_trail(), t, f"trail after step_render {k}")
+            if s.pheromone_diffusion_rate > 0 and k == 0:
+                # the post-diffuse image (slm_render_rgba8) is a different one
+                assert not np.array_equal(sim.render(lut, tw, th), got)
+
+
+def test_step_render_rejects_bad_arguments(slm):
+    with slm.Simulation(32, 32, 10) as sim:
+        L = slm.lib()
+        lut = slm.simulation._lut_bytes(slm.lut.gray())
+        out = (ctypes.c_uint8 * 64)()
+        assert L.slm_step_render_rgba8(sim.handle, 0, lut, 4, 4, out) == -1
+        assert L.slm_step_render_rgba8(sim.handle, 1, None, 4, 4, out) == -1
+        assert L.slm_step_render_rgba8_device(sim.handle, 1, lut, 0, 4, out) == -1
+        assert sim.info().steps_done == 0
 
 
 def test_render_rejects_bad_arguments(slm):

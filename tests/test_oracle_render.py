@@ -59,3 +59,35 @@ def test_non_integer_fit_samples_nearest_lower_cell(orc):
     row = img[1, :, 0]
     want = (t[0, (np.arange(25) / 2.5).astype(int)] * np.float32(255)).astype(np.uint8)
     assert np.array_equal(row, want)
+
+
+def test_display_point_is_the_step_without_its_diffuse(orc, slm):
+    """The reference displays its trail between the decay and the diffuse
+    (src/main.rs:1183-1217).  parity_util.display_point restates that point
+    as the trail pass at diffusion rate 0 over the moved agents' cells; with
+    the rate already 0 it must equal the whole oracle step's trail, and with
+    decay 0 too it is the old trail with exactly the agents' truncated cells
+    raised by the deposit."""
+    from parity_util import display_point, preset
+
+    rng = np.random.default_rng(11)
+    w, h, n = 97, 61, 700
+    trail = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    for name in ("Default", "Firecracker Trees", "Snake"):
+        s = preset(slm, name).clone()
+        s.pheromone_diffusion_rate = 0.0
+        a = orc.init_agents(5, w, h, n, s.agent_speed_min, s.agent_speed_max)
+        a1, t1 = a.copy(), trail.copy()
+        shown = display_point(orc, a1, t1, s)
+        a2, t2 = a.copy(), trail.copy()
+        orc.step(a2, t2, s, 1)
+        assert np.array_equal(shown, t2) and np.array_equal(a1, a2), name
+        s.pheromone_decay_factor = 0.0
+        a3, t3 = a.copy(), trail.copy()
+        shown = display_point(orc, a3, t3, s)
+        want = trail.copy()
+        xs = np.floor(a3[:, 0]).astype(np.int64)
+        ys = np.floor(a3[:, 1]).astype(np.int64)
+        ok = (xs < w) & (ys < h)
+        want[ys[ok], xs[ok]] = np.minimum(want[ys[ok], xs[ok]] + np.float32(1.0), np.float32(1.0))
+        assert np.array_equal(shown, want), name
