@@ -372,6 +372,21 @@ int slm_render_rgba8_device(slm_sim* sim, const uint8_t* lut, uint32_t tex_w, ui
                             void* rgba_device);
 int slm_batch_render_rgba8(slm_batch* batch, uint32_t index, const uint8_t* lut, uint32_t tex_w,
                            uint32_t tex_h, uint8_t* rgba);
+/* slm_render_rgba8 without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (a display loop that
+ * renders every step, code under a harness that forbids hipMalloc).  It does
+ * what slm_render_rgba8 does, with the same bytes, but stages the image in a
+ * fixed per-GPU device array of SLM_RENDER_NOALLOC_MAX_BYTES (16 MiB, so
+ * tex_w * tex_h <= 4194304 pixels; larger textures return
+ * SLM_ERR_INVALID_ARGUMENT) that the library holds until the process ends,
+ * instead of a buffer allocated and freed per call.  Every call on one GPU
+ * shares that array and keeps the last image in it (each call writes every
+ * pixel it returns, so nothing needs clearing): calls that overlap on one GPU
+ * - from different threads, whatever the handle - race, and the caller must
+ * serialise them.  Waits for the image. */
+#define SLM_RENDER_NOALLOC_MAX_BYTES (16u << 20)
+int slm_render_rgba8_noalloc(slm_sim* sim, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
+                             uint8_t* rgba);
 /* The calls above show the trail after the last step's diffuse.  The
  * reference's frame displays it earlier: agents, decay, display, diffuse
  * (src/main.rs:1163-1235; the display pass at :1199-1215 samples the trail

@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2173,6 +2174,59 @@ int slm_render_rgba8_device(slm_sim* s, const uint8_t* lut, uint32_t tex_w, uint
   SIM_GUARD(s);
   SLM_TRY(check_render(lut, tex_w, tex_h, rgba_device));
   return s->eng->render(0, lut, tex_w, tex_h, rgba_device, true);
+  SLM_ENTRY_END
+}
+
+// ---- rendering without allocation ----------------------------------------------
+// slm_render_rgba8 stages its image in a device buffer it allocates and frees
+// on every call.  slm_render_rgba8_noalloc stages it here instead: one fixed
+// array per GPU, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory, so it is only ever used through the
+// address hipGetSymbolAddress returns for the current device, looked up once
+// per device (with that device current) and kept in g_scratch_ptr.
+static __device__ __align__(256) uint32_t scratch[SLM_RENDER_NOALLOC_MAX_BYTES / 4];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// This device's copy of `scratch`; `dev` must be the current device.
+int scratch_on(int dev, void** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(SLM_ERR_UNSUPPORTED, "device %d: no staging array beyond device %d", dev,
+                kMaxScratchDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(SLM_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                  hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SLM_OK;
+}
+}  // namespace
+
+int slm_render_rgba8_noalloc(slm_sim* s, const uint8_t* lut, uint32_t tex_w, uint32_t tex_h,
+                             uint8_t* rgba) {
+  SLM_ENTRY_BEGIN
+  SIM_GUARD(s);
+  SLM_TRY(check_render(lut, tex_w, tex_h, rgba));
+  if (static_cast<uint64_t>(tex_w) * tex_h * 4u > SLM_RENDER_NOALLOC_MAX_BYTES)
+    return fail(SLM_ERR_INVALID_ARGUMENT,
+                "texture of %ux%u needs %llu bytes; slm_render_rgba8_noalloc stages at most "
+                "%u (SLM_RENDER_NOALLOC_MAX_BYTES)",
+                tex_w, tex_h, static_cast<unsigned long long>(tex_w) * tex_h * 4u,
+                static_cast<unsigned>(SLM_RENDER_NOALLOC_MAX_BYTES));
+  void* staging = nullptr;
+  SLM_TRY(scratch_on(s->eng->cfg.device, &staging));
+  return s->eng->render_via(0, lut, tex_w, tex_h, rgba, staging);
   SLM_ENTRY_END
 }
 

@@ -202,6 +202,8 @@ SIGNATURES = {
                                 ctypes.c_void_p],
     "slm_batch_render_rgba8": [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
                                ctypes.c_uint32, ctypes.c_void_p],
+    "slm_render_rgba8_noalloc": [_P, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_void_p],
     "slm_step_render_rgba8": [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
                               ctypes.c_uint32, ctypes.c_void_p],
     "slm_step_render_rgba8_device": [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
