@@ -87,18 +87,19 @@ typedef enum slm_storage {
 
 /* Creation-time configuration.  Start from slm_config_default(). */
 typedef struct slm_config {
-  uint32_t width;           /* grid width in cells  (>= 1) */
-  uint32_t height;          /* grid height in cells (>= 1); width*height <= 2^31 */
+  uint32_t width;           /* grid width in cells  (1 .. 2^24 - 1) */
+  uint32_t height;          /* grid height in cells (1 .. 2^24 - 1); width*height <= 2^31 */
   uint64_t agent_count;     /* >= 0, < 2^32 */
   uint64_t seed;            /* agent initialisation seed (SplitMix64) */
   int32_t storage;          /* slm_storage */
   int32_t device;           /* HIP device ordinal */
   uint32_t resort_interval; /* Morton re-sort of agents every k steps; 0 = never.
-                               Inside a captured chunk the re-sorts fall on the
-                               chunk's steps 0, k, 2k, ..., so a chunk length
-                               that k does not divide re-sorts a little more
-                               often.  Results do not depend on when re-sorts
-                               happen. */
+                               The re-sorts fall on the simulation's steps 0, k,
+                               2k, ... however the steps are split into calls
+                               and whether they run eagerly or from captured
+                               graphs (a captured chunk is kept per starting
+                               point in that schedule).  Results do not depend
+                               on when re-sorts happen. */
   uint32_t graph_chunk;     /* steps per captured hipGraph; 0 = launch eagerly */
   uint32_t sort_cell_log2;  /* re-sort bin edge = 2^k cells (default 4) */
   uint32_t flags;           /* SLM_FLAG_* */
@@ -137,6 +138,20 @@ typedef struct slm_config {
  * each sum starting from its first product and adding every further tap with
  * one fused multiply-add (a single rounding); mix as the 3x3 pass. */
 #define SLM_FLAG_GAUSSIAN_DIFFUSE 0x10u
+/* Windowed sensing: each 256-agent workgroup of the agent pass stages the
+ * trail around its agents' bounding box, widened by the sensor reach, in LDS
+ * (at most 31 KiB) and senses from there; taps outside the window, and
+ * workgroups whose window would not fit, read global memory, so results are
+ * unchanged.  Pays off after Morton re-sorts, which keep a workgroup's agents
+ * together.  Exclusive with SLM_FLAG_TILED_SENSING. */
+#define SLM_FLAG_WINDOW_SENSING 0x20u
+/* Persistent passes: the agent and trail passes of a step each launch only
+ * the workgroups the GPU holds at once, which claim the pass's work from
+ * device counters (reset by the kernels themselves, so captured graphs replay
+ * them) until none is left.  One agent launch and one trail launch per step
+ * over every instance, as without the flag; not with instance groups
+ * (SLM_ERR_UNSUPPORTED from the tuning calls).  Results are unchanged. */
+#define SLM_FLAG_PERSISTENT 0x40u
 
 typedef struct slm_info {
   uint32_t width, height;
@@ -301,7 +316,8 @@ typedef struct slm_tuning {
   uint32_t trail_rows;       /* rows of one trail-pass tile (64*vector-width
                                 columns): 4, 8, 16, 32 or 64; 0 = chosen from the
                                 grid so the launch has >= 8192 waves */
-  uint32_t tiled_sensing;    /* 1 = SLM_FLAG_TILED_SENSING (needs re-sorting) */
+  uint32_t tiled_sensing;    /* 1 = SLM_FLAG_TILED_SENSING (needs re-sorting),
+                                2 = SLM_FLAG_WINDOW_SENSING, 0 = neither */
   uint32_t trials;           /* autotune: configurations timed; else 0 */
   uint32_t instance_groups;  /* 1..4096 (0 reads as 1): between re-sorts the
                                 instances step as this many contiguous groups
@@ -332,7 +348,7 @@ int slm_set_tuning(slm_sim* sim, const slm_tuning* tuning);
 /* Times candidate tunings on the simulation's own state and keeps the
  * fastest: the trail tile rows by trail-pass time, streaming (0..3), then the
  * re-sort interval
- * (0, 5, 10, 20 or 40), the bin size, tiled sensing, agents per workgroup and
+ * (0, 5, 10, 20 or 40), the bin size, the sensing mode (0, 1, 2), agents per workgroup and
  * (batches) instance groups (powers of 4 up to the instance count, then the
  * best one's neighbours x2 and /2) by total step time (coordinate descent).
  * Step times are spans over steps launched as slm_step launches them; kernel

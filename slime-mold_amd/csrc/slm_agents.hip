@@ -24,6 +24,8 @@
 // double: sensor offsets and the move step are still the shader's f32 products,
 // added to the f64 position; the wrap, floor and deposit cell are taken in
 // double; the bilinear weights and the jitter hash use the f32 roundings.
+#include <type_traits>
+
 #include "slm_internal.h"
 #include "slm_trig.h"
 
@@ -143,7 +145,8 @@ __device__ inline float sample_at(const void* t, uint32_t w, uint32_t h, int x0,
                                   float dy) {
   bool in = x0 >= 0 && static_cast<uint32_t>(x0) < w - 1u && y0 >= 0 &&
             static_cast<uint32_t>(y0) < h - 1u;
-  uint32_t i00 = in ? static_cast<uint32_t>(y0) * w + static_cast<uint32_t>(x0) : 0u;
+  // grid sides are below 2^24, so the row offset is a full-rate 24-bit product
+  uint32_t i00 = in ? __umul24(static_cast<uint32_t>(y0), w) + static_cast<uint32_t>(x0) : 0u;
   uint32_t down = in ? w : 0u;
   float v00, v10, v01, v11;
   TrailRead<S>::load_pair(t, i00, v00, v10);
@@ -162,7 +165,7 @@ struct GlobalTaps {
   __device__ float sample(int x0, int y0, float dx, float dy) const {
     return sample_at<S>(t, w, h, x0, y0, dx, dy);
   }
-  __device__ void mark(uint32_t x, uint32_t y) const { TrailRead<S>::mark(t, y * w + x); }
+  __device__ void mark(uint32_t x, uint32_t y) const { TrailRead<S>::mark(t, __umul24(y, w) + x); }
 };
 
 // Taps from a window of the trail staged in LDS (|T| values of cells
@@ -188,7 +191,7 @@ struct WindowTaps {
       v01 = TrailRead<S>::value(win[i + ww]);
       v11 = TrailRead<S>::value(win[i + ww + 1]);
     } else {
-      uint32_t i00 = in ? static_cast<uint32_t>(y0) * w + static_cast<uint32_t>(x0) : 0u;
+      uint32_t i00 = in ? __umul24(static_cast<uint32_t>(y0), w) + static_cast<uint32_t>(x0) : 0u;
       uint32_t down = in ? w : 0u;
       TrailRead<S>::load_pair(t, i00, v00, v10);
       TrailRead<S>::load_pair(t, i00 + down, v01, v11);
@@ -202,10 +205,76 @@ struct WindowTaps {
   __device__ void mark(uint32_t x, uint32_t y) const {
     int lx = static_cast<int>(x) - wx0, ly = static_cast<int>(y) - wy0;
     if (lx >= 0 && lx < ww && ly >= 0 && ly < wh)
-      static_cast<elem*>(t)[y * w + x] =
+      static_cast<elem*>(t)[__umul24(y, w) + x] =
           static_cast<elem>(static_cast<uint32_t>(win[ly * ww + lx]) | TrailRead<S>::kSign);
     else
-      TrailRead<S>::mark(t, y * w + x);
+      TrailRead<S>::mark(t, __umul24(y, w) + x);
+  }
+};
+
+// Taps from a workgroup's window of the trail staged in LDS (windowed
+// sensing, below): the raw elements of cells [wx0, wx0 + ww) x [wy0, wy0 + wh)
+// at `pitch` cells per row (a whole number of 16-byte chunks; marks included,
+// so values are read as |T|).  Taps outside it go to global memory, so the
+// result does not depend on the window.
+template <int S>
+struct BoxTaps {
+  using elem = typename TrailRead<S>::elem;
+  void* t;
+  uint32_t w, h;
+  const uint32_t* win;  // LDS
+  int wx0, wy0, ww, wh, pitch;
+  // |T| of cells i and i + 1 of the window (element index i)
+  __device__ void pair(int i, float& a, float& b) const {
+    if constexpr (sizeof(elem) == 4) {
+      a = __uint_as_float(win[i] & 0x7fffffffu);
+      b = __uint_as_float(win[i + 1] & 0x7fffffffu);
+    } else {
+      // the two dwords around cells i, i + 1; the 32 bits starting at cell i
+      const uint32_t lo = win[i >> 1], hi = win[(i >> 1) + 1];
+      const uint32_t v = __builtin_amdgcn_alignbit(hi, lo, (i & 1) * 16);
+      a = TrailRead<S>::value(v & 0x7fffu);
+      b = TrailRead<S>::value((v >> 16) & 0x7fffu);
+    }
+  }
+  __device__ float sample(int x0, int y0, float dx, float dy) const {
+    bool in = x0 >= 0 && static_cast<uint32_t>(x0) < w - 1u && y0 >= 0 &&
+              static_cast<uint32_t>(y0) < h - 1u;
+    const int lx = x0 - wx0, ly = y0 - wy0;
+    const bool inw = static_cast<uint32_t>(lx) < static_cast<uint32_t>(ww - 1) &&
+                     static_cast<uint32_t>(ly) < static_cast<uint32_t>(wh - 1);
+    float v00, v10, v01, v11;
+    if (inw) {
+      const int i = __mul24(ly, pitch) + lx;  // both below 2^16
+      pair(i, v00, v10);
+      pair(i + pitch, v01, v11);
+    } else {
+      // sides below 2^16 here, so the row offset is a 24-bit product
+      uint32_t i00 = in ? __umul24(static_cast<uint32_t>(y0), w) + static_cast<uint32_t>(x0) : 0u;
+      uint32_t down = in ? w : 0u;
+      TrailRead<S>::load_pair(t, i00, v00, v10);
+      TrailRead<S>::load_pair(t, i00 + down, v01, v11);
+    }
+    float v0 = wgsl_mix(v00, v10, dx);
+    float v1 = wgsl_mix(v01, v11, dx);
+    float v = wgsl_mix(v0, v1, dy);
+    return in ? v : 0.0f;
+  }
+  // A cell inside the window: |T| (or -|T|) is known, so the mark is a store.
+  __device__ void mark(uint32_t x, uint32_t y) const {
+    const int lx = static_cast<int>(x) - wx0, ly = static_cast<int>(y) - wy0;
+    if (static_cast<uint32_t>(lx) < static_cast<uint32_t>(ww) &&
+        static_cast<uint32_t>(ly) < static_cast<uint32_t>(wh)) {
+      const int i = __mul24(ly, pitch) + lx;
+      uint32_t raw;
+      if constexpr (sizeof(elem) == 4)
+        raw = win[i];
+      else
+        raw = (win[i >> 1] >> ((i & 1) * 16)) & 0xffffu;
+      static_cast<elem*>(t)[__umul24(y, w) + x] = static_cast<elem>(raw | TrailRead<S>::kSign);
+    } else {
+      TrailRead<S>::mark(t, __umul24(y, w) + x);
+    }
   }
 };
 
@@ -291,9 +360,10 @@ __device__ inline void agent_update(const AgentSoA& a, uint32_t j, const AgentSt
 
   float al = angle - p.sensor_angle, ar = angle + p.sensor_angle;  // :75-77
   float sl, cl, sr, cr, sc, cc;
-  slm_sincosf(al, &sl, &cl);
-  slm_sincosf(ar, &sr, &cr);
-  slm_sincosf(angle, &sc, &cc);
+  // every argument here is finite: positions, angles and parameters are
+  slm_sincosf_finite(al, &sl, &cl);
+  slm_sincosf_finite(ar, &sr, &cr);
+  slm_sincosf_finite(angle, &sc, &cc);
   float sd = p.sensor_dist;
   float lv, rv, cv;
   if (P64) {
@@ -319,7 +389,7 @@ __device__ inline void agent_update(const AgentSoA& a, uint32_t j, const AgentSt
 
   float move = speed * kTimeStep;  // :125-127
   float ms, mc;
-  slm_sincosf(angle, &ms, &mc);
+  slm_sincosf_finite(angle, &ms, &mc);
   int dx, dy;
   if (P64) {
     double fw = static_cast<double>(w), fh = static_cast<double>(h);
@@ -387,20 +457,139 @@ struct AgentArgs {
   uint32_t count, total;
   uint32_t group;  // agents per workgroup, a multiple of kAgentBlock
   int xcd_remap;
+  uint32_t* claim;  // persistent launch: claim counters (nullptr: one workgroup per block)
+  uint32_t blocks;  // logical workgroups of the pass
 };
 
-// The tables are separate restrict-qualified arguments: nothing the kernel
-// stores aliases them, so their block-uniform loads become scalar loads.
+// ---- windowed sensing ----------------------------------------------------------
+// After a Morton re-sort the 256 agents of a workgroup sit in a few
+// neighbouring bins.  The workgroup takes the bounding box of their cells,
+// widens it by the sensor reach (ceil|sensor distance| + 2 cells covers every
+// bilinear tap) and stages that window of the trail in LDS with 16-byte row
+// loads, then senses from LDS: one coalesced pass over the window instead of
+// six divergent tap-pair loads per agent, which keep the texture addresser
+// busy.  Taps outside the window still read global memory, so results do not
+// depend on the window.  A window beyond the LDS budget (agents spread out,
+// sensor distances in the hundreds) senses from global memory for the whole
+// workgroup.
+// 31 KiB (plus 16 bytes of padding for the 16-bit reads' second dword) keeps
+// five workgroups resident per CU in gfx950's 160 KiB of LDS.
+constexpr uint32_t kWinBytes = 31744;
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                 __builtin_bit_cast(u16x2, b)));
+}
+
+// One workgroup of kAgentBlock slots [first, first + 256) inside instance
+// `in` (whose sides are below 2^16 and whose rows are whole 16-byte chunks);
+// thread j's agent when j <= last.
 template <int S, bool P64>
-__global__ __launch_bounds__(kAgentBlock) void agents_kernel(
-    AgentArgs args, const Instance* __restrict__ insts, const DevParams* __restrict__ params,
-    const uint32_t* __restrict__ block_inst) {
-  uint32_t blk = args.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+__device__ inline void window_block(const AgentArgs& args, uint32_t j, uint32_t last,
+                                    const Instance& in, const DevParams& p, uint32_t* s_win,
+                                    uint32_t* s_box) {
+  using elem = typename TrailRead<S>::elem;
+  constexpr int V = static_cast<int>(16 / sizeof(elem));
+  elem* trail = static_cast<elem*>(args.trail) + in.trail_off;
+  const bool live = j <= last;
+  AgentState<P64> st{};
+  if (live) st = load_agent<P64>(args.a, j);
+  // bounding box of the agents' cells: per-component u16 maxima of (x | y << 16)
+  // and of its complement (the minima), over the wave, then the workgroup
+  uint32_t cx = 0, cy = 0;
+  if (live) {
+    cx = static_cast<uint32_t>(P64 ? static_cast<int>(st.xd) : static_cast<int>(st.x));
+    cy = static_cast<uint32_t>(P64 ? static_cast<int>(st.yd) : static_cast<int>(st.y));
+  }
+  // wave maxima by DPP row operations (no LDS round trips)
+  const uint32_t mx = __ockl_wfred_max_u32(live ? cx : 0u);
+  const uint32_t my = __ockl_wfred_max_u32(live ? cy : 0u);
+  const uint32_t nx = __ockl_wfred_max_u32(live ? 0xffffu - cx : 0u);
+  const uint32_t ny = __ockl_wfred_max_u32(live ? 0xffffu - cy : 0u);
+  uint32_t hi = (my << 16) | mx;
+  uint32_t lo = (ny << 16) | nx;
+  const int wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
+  if (lane == 0) {
+    s_box[2 * wave] = hi;
+    s_box[2 * wave + 1] = lo;
+  }
+  __syncthreads();
+  hi = pk_max_u16(pk_max_u16(s_box[0], s_box[2]), pk_max_u16(s_box[4], s_box[6]));
+  lo = pk_max_u16(pk_max_u16(s_box[1], s_box[3]), pk_max_u16(s_box[5], s_box[7]));
+  const int W = static_cast<int>(in.width), H = static_cast<int>(in.height);
+  const int reach = static_cast<int>(__builtin_ceilf(__builtin_fabsf(p.sensor_dist))) + 2;
+  const int minx = 0xffff - static_cast<int>(lo & 0xffffu), maxx = static_cast<int>(hi & 0xffffu);
+  const int miny = 0xffff - static_cast<int>(lo >> 16), maxy = static_cast<int>(hi >> 16);
+  const int wx0 = max(minx - reach, 0) & ~(V - 1);
+  const int wy0 = max(miny - reach, 0);
+  const int ww = min(maxx + reach + 1, W) - wx0;
+  const int wh = min(maxy + reach + 1, H) - wy0;
+  const int cpr = (ww + V - 1) / V;  // 16-byte chunks per window row
+  const int pitch = cpr * V;
+  const bool staged = cpr <= 64 && static_cast<uint32_t>(pitch) * static_cast<uint32_t>(wh) *
+                                           sizeof(elem) <= kWinBytes;  // uniform
+  if (!staged) {
+    GlobalTaps<S> taps{trail, in.width, in.height};
+    if (live) agent_update<P64>(args.a, j, st, in, p, taps);
+    return;
+  }
+  // Stage: lane l takes chunk l % cw of sub-row l / cw (cw = the power of two
+  // >= cpr), so a wave covers 64 / cw rows per pass; four loads in flight per
+  // lane before their LDS stores.
+  const int cw = cpr <= 1 ? 1 : 1 << (32 - __builtin_clz(static_cast<uint32_t>(cpr - 1)));
+  const int per = 64 / cw;  // rows per wave pass
+  const int c = lane & (cw - 1);
+  const int step = 4 * per;  // rows per workgroup pass
+  const elem* src = trail + static_cast<size_t>(wy0) * in.width + wx0 + c * V;
+  unsigned char* dst = reinterpret_cast<unsigned char*>(s_win) + c * 16;
+  const bool col_ok = c < cpr;
+  auto load = [&](int r) {
+    return *reinterpret_cast<const uint4*>(src + static_cast<size_t>(r) * in.width);
+  };
+  auto store = [&](int r, uint4 v) {
+    *reinterpret_cast<uint4*>(dst + static_cast<size_t>(r) * pitch * sizeof(elem)) = v;
+  };
+  for (int r0 = wave * per + lane / cw; r0 < wh; r0 += 4 * step) {
+    const int r1 = r0 + step, r2 = r0 + 2 * step, r3 = r0 + 3 * step;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    const uint4 v0 = col_ok ? load(r0) : z;
+    const uint4 v1 = col_ok && r1 < wh ? load(r1) : z;
+    const uint4 v2 = col_ok && r2 < wh ? load(r2) : z;
+    const uint4 v3 = col_ok && r3 < wh ? load(r3) : z;
+    if (col_ok) store(r0, v0);
+    if (col_ok && r1 < wh) store(r1, v1);
+    if (col_ok && r2 < wh) store(r2, v2);
+    if (col_ok && r3 < wh) store(r3, v3);
+  }
+  __syncthreads();
+  BoxTaps<S> taps{trail, in.width, in.height, s_win, wx0, wy0, ww, wh, pitch};
+  if (live) agent_update<P64>(args.a, j, st, in, p, taps);
+}
+
+// One logical workgroup `blk` of the agent pass.  kWin: one-agent-per-thread
+// workgroups that lie inside one instance sense through an LDS window
+// (window_block).
+template <int S, bool P64, bool kWin>
+__device__ inline void agents_block(const AgentArgs& args, const Instance* __restrict__ insts,
+                                    const DevParams* __restrict__ params,
+                                    const uint32_t* __restrict__ block_inst, uint32_t blk,
+                                    uint32_t* s_win, uint32_t* s_box) {
   using elem = typename TrailRead<S>::elem;
   if (args.group == kAgentBlock) {  // one agent per thread
     uint32_t first = blk * kAgentBlock;
     uint32_t last = min(first + kAgentBlock, args.total) - 1u;
     uint32_t j = first + threadIdx.x;
+    if constexpr (kWin) {
+      constexpr uint32_t V = 16 / sizeof(elem);
+      const uint32_t ii = block_inst[blk];
+      const Instance in = insts[ii];
+      if (first >= in.agent_off && last < in.agent_off + in.agent_count && in.width < 65536u &&
+          in.height < 65536u && in.width % V == 0u) {  // uniform
+        window_block<S, P64>(args, j, last, in, params[ii], s_win, s_box);
+        return;
+      }
+    }
     // instances overlapping this block, one at a time (block-uniform)
     for (uint32_t ii = block_inst[blk];; ii++) {
       const Instance in = insts[ii];
@@ -429,6 +618,26 @@ __global__ __launch_bounds__(kAgentBlock) void agents_kernel(
     }
     if (stop <= end || ii + 1u >= args.count) break;
   }
+}
+
+// The tables are separate restrict-qualified arguments: nothing the kernel
+// stores aliases them, so their block-uniform loads become scalar loads.
+// With args.claim set the launch is persistent (persistent_blocks).
+template <int S, bool P64, bool kWin>
+__global__ __launch_bounds__(kAgentBlock) void agents_kernel(
+    AgentArgs args, const Instance* __restrict__ insts, const DevParams* __restrict__ params,
+    const uint32_t* __restrict__ block_inst) {
+  __shared__ __align__(16) uint32_t s_win[kWin ? kWinBytes / 4 + 4 : 1];
+  __shared__ uint32_t s_box[8];
+  __shared__ uint32_t s_claim;
+  if (!args.claim) {
+    const uint32_t blk = args.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    agents_block<S, P64, kWin>(args, insts, params, block_inst, blk, s_win, s_box);
+    return;
+  }
+  persistent_blocks(args.claim, args.blocks, &s_claim, [&](uint32_t blk) {
+    agents_block<S, P64, kWin>(args, insts, params, block_inst, blk, s_win, s_box);
+  });
 }
 
 // ---- tiled sensing -----------------------------------------------------------
@@ -640,15 +849,29 @@ __global__ void unpack_positions64_kernel(AgentSoA a, uint32_t off, uint32_t n,
   a.y64[j] = v.y;
 }
 
+template <int S, bool P64, bool kWin>
+void launch_agents_k(const AgentArgs& args, hipStream_t stream) {
+  uint32_t grid = args.claim ? persistent_grid(agents_kernel<S, P64, kWin>, args.blocks,
+                                               kAgentBlock)
+                             : args.blocks;
+  agents_kernel<S, P64, kWin><<<grid, kAgentBlock, 0, stream>>>(args, args.inst, args.params,
+                                                               args.block_inst);
+}
+
+template <int S, bool kWin>
+void launch_agents_w(const Pool& pool, const AgentArgs& args, hipStream_t stream) {
+  if (pool.pos64)
+    launch_agents_k<S, true, kWin>(args, stream);
+  else
+    launch_agents_k<S, false, kWin>(args, stream);
+}
+
 template <int S>
 void launch_agents_t(const Pool& pool, const AgentArgs& args, hipStream_t stream) {
-  uint32_t nb = blocks_for(pool.total_agents, args.group);
-  if (pool.pos64)
-    agents_kernel<S, true><<<nb, kAgentBlock, 0, stream>>>(args, args.inst, args.params,
-                                                           args.block_inst);
+  if (pool.window_sensing && args.group == kAgentBlock)
+    launch_agents_w<S, true>(pool, args, stream);
   else
-    agents_kernel<S, false><<<nb, kAgentBlock, 0, stream>>>(args, args.inst, args.params,
-                                                            args.block_inst);
+    launch_agents_w<S, false>(pool, args, stream);
 }
 
 }  // namespace
@@ -657,7 +880,8 @@ void launch_agents(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t
   if (pool.total_agents == 0) return;
   uint32_t group = pool.agents_per_group ? pool.agents_per_group : kAgentBlock;
   AgentArgs args{a,          trail,      pool.inst,         pool.params, pool.agent_block_inst,
-                 pool.count, pool.total_agents, group, pool.xcd_remap};
+                 pool.count, pool.total_agents, group, pool.xcd_remap, pool.claim_agents,
+                 blocks_for(pool.total_agents, group)};
   switch (pool.storage) {
     case SLM_STORAGE_BF16:
       launch_agents_t<SLM_STORAGE_BF16>(pool, args, stream);
@@ -672,6 +896,19 @@ void launch_agents(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t
 }
 
 uint32_t window_side(int storage) { return window_side_of(storage_bytes(storage)); }
+
+void prime_persistent_agents() {
+  auto each = [](auto s_tag) {
+    constexpr int S = decltype(s_tag)::value;
+    (void)persistent_grid(agents_kernel<S, false, false>, 8, kAgentBlock);
+    (void)persistent_grid(agents_kernel<S, false, true>, 8, kAgentBlock);
+    (void)persistent_grid(agents_kernel<S, true, false>, 8, kAgentBlock);
+    (void)persistent_grid(agents_kernel<S, true, true>, 8, kAgentBlock);
+  };
+  each(std::integral_constant<int, SLM_STORAGE_F32>{});
+  each(std::integral_constant<int, SLM_STORAGE_BF16>{});
+  each(std::integral_constant<int, SLM_STORAGE_F16>{});
+}
 
 void launch_agents_tiled(const Pool& pool, const AgentSoA& a, void* trail, hipStream_t stream) {
   if (pool.total_agents == 0 || pool.max_items == 0) return;

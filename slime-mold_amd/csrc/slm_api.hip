@@ -80,7 +80,10 @@ int fail_exception() noexcept {
     if (rc_ != SLM_OK) return rc_; \
   } while (0)
 
-constexpr uint32_t kMaxSide = 1u << 24;          // f32(width) exact
+// Grid sides below 2^24: f32(width) is exact, and a row offset y * width with
+// y < height is a 24-bit product (full-rate v_mul_u32_u24 in the agent pass).
+constexpr uint32_t kMaxSide = (1u << 24) - 1u;
+constexpr uint32_t kMaxTexSide = 1u << 24;
 constexpr uint64_t kMaxCells = 1ull << 31;        // i32 cell index (compute.wgsl:21)
 constexpr uint64_t kMaxPoolCells = 1ull << 40;
 constexpr uint64_t kMaxAgents = 0xffffff00ull;    // u32 slot index, whole blocks
@@ -90,7 +93,8 @@ constexpr float kMaxTrailValue = 0x1p100f;        // keeps every 3x3 window sum 
 constexpr uint32_t kTargetWaves = 8192;           // trail-pass units per pool
 constexpr uint32_t kKnownFlags = SLM_FLAG_NO_XCD_REMAP | SLM_FLAG_F64_POSITIONS |
                                  SLM_FLAG_TILED_SENSING | SLM_FLAG_STOCHASTIC_ROUNDING |
-                                 SLM_FLAG_GAUSSIAN_DIFFUSE;
+                                 SLM_FLAG_GAUSSIAN_DIFFUSE | SLM_FLAG_WINDOW_SENSING |
+                                 SLM_FLAG_PERSISTENT;
 
 bool finite_params(const slm_params& p) {
   const float* f = &p.decay_factor;
@@ -115,7 +119,7 @@ int check_params(const slm_params* p, uint32_t flags) {
 int check_grid(uint32_t w, uint32_t h) {
   if (w == 0 || h == 0) return fail(SLM_ERR_INVALID_ARGUMENT, "width and height must be >= 1");
   if (w > kMaxSide || h > kMaxSide)
-    return fail(SLM_ERR_INVALID_ARGUMENT, "width and height must be <= 2^24");
+    return fail(SLM_ERR_INVALID_ARGUMENT, "width and height must be < 2^24");
   if (static_cast<uint64_t>(w) * h > kMaxCells)
     return fail(SLM_ERR_INVALID_ARGUMENT, "width*height must be <= 2^31");
   return SLM_OK;
@@ -157,11 +161,11 @@ int check_tuning(const slm_tuning& t) {
   uint32_t r = t.trail_rows;
   if (r != 0 && (r < 4 || r > 64 || (r & (r - 1u))))
     return fail(SLM_ERR_INVALID_ARGUMENT, "trail_rows must be 0, 4, 8, 16, 32 or 64");
-  if (t.tiled_sensing > 1) return fail(SLM_ERR_INVALID_ARGUMENT, "tiled_sensing must be 0 or 1");
+  if (t.tiled_sensing > 2) return fail(SLM_ERR_INVALID_ARGUMENT, "tiled_sensing must be 0, 1 or 2");
   if (t.instance_groups > kMaxInstanceGroups)
     return fail(SLM_ERR_INVALID_ARGUMENT, "instance_groups must be <= %u", kMaxInstanceGroups);
   if (t.streaming > 3) return fail(SLM_ERR_INVALID_ARGUMENT, "streaming must be 0..3");
-  if (t.tiled_sensing && t.instance_groups > 1)
+  if (t.tiled_sensing == 1 && t.instance_groups > 1)
     return fail(SLM_ERR_UNSUPPORTED, "instance groups do not combine with tiled sensing");
   return SLM_OK;
 }
@@ -174,6 +178,8 @@ int check_engine_config(const EngineConfig& c, const uint32_t* reserved, size_t 
   if (c.flags & ~kKnownFlags) return fail(SLM_ERR_INVALID_ARGUMENT, "unknown flags");
   if ((c.flags & SLM_FLAG_STOCHASTIC_ROUNDING) && c.storage != SLM_STORAGE_BF16)
     return fail(SLM_ERR_UNSUPPORTED, "stochastic rounding needs bf16 storage");
+  if ((c.flags & SLM_FLAG_TILED_SENSING) && (c.flags & SLM_FLAG_WINDOW_SENSING))
+    return fail(SLM_ERR_INVALID_ARGUMENT, "tiled and windowed sensing are exclusive");
   for (size_t i = 0; i < nreserved; i++)
     if (reserved[i]) return fail(SLM_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
   return SLM_OK;
@@ -335,6 +341,10 @@ class Engine {
     p.stochastic_rounding = (cfg.flags & SLM_FLAG_STOCHASTIC_ROUNDING) ? 1 : 0;
     p.streaming = static_cast<int>(cfg.streaming);
     p.gaussian = (cfg.flags & SLM_FLAG_GAUSSIAN_DIFFUSE) ? 1 : 0;
+    p.window_sensing = (cfg.flags & SLM_FLAG_WINDOW_SENSING) ? 1 : 0;
+    const bool persistent = (cfg.flags & SLM_FLAG_PERSISTENT) != 0;
+    p.claim_agents = persistent ? claim_d_ : nullptr;
+    p.claim_trail = persistent ? claim_d_ + kClaimWords : nullptr;
     p.total_gtiles = total_gtiles_;
     return p;
   }
@@ -684,7 +694,9 @@ class Engine {
     t.sort_cell_log2 = cfg.sort_cell_log2;
     t.agents_per_group = cfg.agents_per_group;
     t.trail_rows = cfg.trail_rows;
-    t.tiled_sensing = (cfg.flags & SLM_FLAG_TILED_SENSING) ? 1u : 0u;
+    t.tiled_sensing = (cfg.flags & SLM_FLAG_TILED_SENSING)   ? 1u
+                      : (cfg.flags & SLM_FLAG_WINDOW_SENSING) ? 2u
+                                                              : 0u;
     t.instance_groups = cfg.instance_groups;
     t.streaming = cfg.streaming;
     return t;
@@ -692,6 +704,8 @@ class Engine {
 
   // Keeps agents, trail, step count and parameters.
   int set_tuning(const slm_tuning& t) {
+    if ((cfg.flags & SLM_FLAG_PERSISTENT) && t.instance_groups > 1)
+      return fail(SLM_ERR_UNSUPPORTED, "persistent passes do not combine with instance groups");
     SLM_TRY(sync());
     if (!fits(t)) return fail(SLM_ERR_OUT_OF_MEMORY, "not enough device memory for this tuning");
     if (sorting() == (t.resort_interval > 0)) return apply_tuning(t);
@@ -881,10 +895,9 @@ class Engine {
       cfg.trail_rows = t.trail_rows;
       cfg.instance_groups = t.instance_groups ? t.instance_groups : 1u;
       cfg.streaming = t.streaming;
-      if (t.tiled_sensing)
-        cfg.flags |= SLM_FLAG_TILED_SENSING;
-      else
-        cfg.flags &= ~SLM_FLAG_TILED_SENSING;
+      cfg.flags &= ~(SLM_FLAG_TILED_SENSING | SLM_FLAG_WINDOW_SENSING);
+      if (t.tiled_sensing == 1) cfg.flags |= SLM_FLAG_TILED_SENSING;
+      if (t.tiled_sensing == 2) cfg.flags |= SLM_FLAG_WINDOW_SENSING;
       if (new_agents) SLM_TRY(layout_agents());
       return layout_trail_tables();
     }());
@@ -1007,9 +1020,15 @@ class Engine {
         t.sort_cell_log2 = c;
         SLM_TRY(take(t));
       }
-      if (best.instance_groups <= 1) {
+    }
+    // sensing: global taps (0), tile work lists (1: needs re-sorting and one
+    // instance group), per-workgroup LDS windows (2)
+    {
+      const uint32_t cur = best.tiled_sensing;
+      for (uint32_t m : {0u, 1u, 2u}) {
+        if (m == cur || (m == 1u && (!best.resort_interval || best.instance_groups > 1))) continue;
         slm_tuning t = best;
-        t.tiled_sensing ^= 1u;
+        t.tiled_sensing = m;
         SLM_TRY(take(t));
       }
     }
@@ -1024,7 +1043,8 @@ class Engine {
     std::vector<uint32_t> tried_groups;
     auto try_groups = [&](uint32_t g) -> int {
       const uint32_t cap = std::min<uint32_t>(count(), kMaxInstanceGroups);
-      if (best.tiled_sensing || g < 1 || g > cap) return SLM_OK;
+      if (best.tiled_sensing == 1 || (cfg.flags & SLM_FLAG_PERSISTENT) || g < 1 || g > cap)
+        return SLM_OK;
       for (uint32_t x : tried_groups)
         if (x == g) return SLM_OK;
       tried_groups.push_back(g);
@@ -1074,6 +1094,7 @@ class Engine {
   uint32_t* scan_levels_ = nullptr;
   double* mass_partials_ = nullptr;
   double* mass_out_ = nullptr;
+  uint32_t* claim_d_ = nullptr;  // persistent passes: agent counters, then trail counters
   uint32_t total_agents_ = 0, total_units_ = 0, total_bins_ = 0;
   uint32_t total_tiles_ = 0, max_items_ = 0;
   uint32_t total_gtiles_ = 0;  // Gaussian-pass tiles
@@ -1083,7 +1104,18 @@ class Engine {
   uint32_t* item_total_d_ = nullptr;
   bool tiles_ready_ = false;  // the latest re-sort's work list matches the layout
   uint64_t trail_elems_ = 0;
-  hipGraphExec_t graphs_[2][2] = {};
+  // Captured chunks.  A chunk's launches depend on the current trail and
+  // agent buffers and on where the chunk starts in the re-sort schedule
+  // (absolute steps 0, k, 2k, ..., as eager steps follow), so that is the key.
+  struct ChunkKey {
+    int cur, acur;
+    uint32_t phase;  // steps_done % resort_interval at the chunk's start (0 without re-sorts)
+    bool operator==(const ChunkKey& o) const {
+      return cur == o.cur && acur == o.acur && phase == o.phase;
+    }
+  };
+  static constexpr size_t kMaxCachedChunks = 16;  // per kind; the oldest goes first
+  std::vector<std::pair<ChunkKey, hipGraphExec_t>> graphs_;
   std::vector<std::pair<void*, size_t>> allocs_;
 
   int dev_alloc(void** p, size_t bytes) {
@@ -1164,6 +1196,13 @@ class Engine {
     SLM_TRY(dev_alloc(&params_d_, spec.size() * sizeof(DevParams)));
     SLM_TRY(dev_alloc(&mass_partials_, spec.size() * kMassBlocks * sizeof(double)));
     SLM_TRY(dev_alloc(&mass_out_, spec.size() * sizeof(double)));
+    // persistent passes' claim counters: zero here, reset by the kernels after
+    SLM_TRY(dev_alloc(&claim_d_, 2 * kClaimWords * sizeof(uint32_t)));
+    SLM_HIP(hipMemset(claim_d_, 0, 2 * kClaimWords * sizeof(uint32_t)));
+    if (cfg.flags & SLM_FLAG_PERSISTENT) {
+      prime_persistent_agents();
+      prime_persistent_trail();
+    }
     return SLM_OK;
   }
 
@@ -1453,7 +1492,7 @@ class Engine {
   int build_groups() {
     free_groups();
     const uint32_t want = std::min<uint32_t>(cfg.instance_groups, count());
-    if (want <= 1 || (cfg.flags & SLM_FLAG_TILED_SENSING)) return SLM_OK;
+    if (want <= 1 || (cfg.flags & (SLM_FLAG_TILED_SENSING | SLM_FLAG_PERSISTENT))) return SLM_OK;
     std::vector<uint64_t> cost(count());
     uint64_t total = 0;
     for (size_t k = 0; k < count(); k++) {
@@ -1612,32 +1651,50 @@ class Engine {
     }
   }
 
-  bool chunk_sorts(uint32_t j) const {
-    return sorting() && total_agents_ > 0 && j % cfg.resort_interval == 0;
+  // Whether absolute step `step` starts with a re-sort.
+  bool sorts_at(uint64_t step) const { return sort_due(step) && total_agents_ > 0; }
+
+  ChunkKey chunk_key() const {
+    return {cur_, acur_, sorting() ? static_cast<uint32_t>(steps_done % cfg.resort_interval) : 0u};
+  }
+
+  // Cached entry for `key` in `cache`, or nullptr.
+  template <class T>
+  static T* cached(std::vector<std::pair<ChunkKey, T>>& cache, const ChunkKey& key) {
+    for (auto& e : cache)
+      if (e.first == key) return &e.second;
+    return nullptr;
   }
 
   int run_chunk_graph() {
-    uint32_t chunk = cfg.graph_chunk;
-    hipGraphExec_t& exec = graphs_[cur_][acur_];
+    const uint32_t chunk = cfg.graph_chunk;
+    const uint64_t s0 = steps_done;
+    const ChunkKey key = chunk_key();
+    hipGraphExec_t* found = cached(graphs_, key);
+    hipGraphExec_t exec = found ? *found : nullptr;
     if (!exec) {
+      if (graphs_.size() >= kMaxCachedChunks) {  // drop the oldest, once it has run
+        SLM_HIP(hipStreamSynchronize(stream));
+        (void)hipGraphExecDestroy(graphs_.front().second);
+        graphs_.erase(graphs_.begin());
+      }
       int cur0 = cur_, acur0 = acur_;
       hipGraph_t graph = nullptr;
       SLM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      enqueue_steps(chunk, [&](uint32_t j) { return chunk_sorts(j); });
+      enqueue_steps(chunk, [&](uint32_t j) { return sorts_at(s0 + j); });
       hipError_t e = hipStreamEndCapture(stream, &graph);
       cur_ = cur0;
       acur_ = acur0;
       if (e != hipSuccess) return fail(SLM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
       e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) {
-        exec = nullptr;
+      if (e != hipSuccess)
         return fail(SLM_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
-      }
+      graphs_.push_back({key, exec});
     }
     SLM_HIP(hipGraphLaunch(exec, stream));
     for (uint32_t j = 0; j < chunk; j++) {  // the buffer swaps the captured steps make
-      if (chunk_sorts(j)) acur_ ^= 1;
+      if (sorts_at(s0 + j)) acur_ ^= 1;
       cur_ ^= 1;
     }
     steps_done += chunk;
@@ -1732,7 +1789,7 @@ class Engine {
     std::vector<hipEvent_t> ev;
     std::vector<uint8_t> sorted;
   };
-  TimedChunk timed_[2][2];
+  std::vector<std::pair<ChunkKey, TimedChunk>> timed_;
   bool timed_graphs_off_ = false;
 
   // A timed chunk is built node by node: HIP drops event records made during
@@ -1771,7 +1828,7 @@ class Engine {
     };
     for (uint32_t j = 0; j < chunk; j++) {
       hipEvent_t* ev = &tc.ev[static_cast<size_t>(j) * kEv];
-      bool sort = chunk_sorts(j);
+      bool sort = sorts_at(steps_done + j);
       tc.sorted[j] = sort;
       if (sort) {
         add_event(ev[0]);
@@ -1803,8 +1860,20 @@ class Engine {
 
   int run_timed_chunk(slm_timing* out) {
     uint32_t chunk = cfg.graph_chunk;
-    TimedChunk& tc = timed_[cur_][acur_];
-    if (!tc.exec) SLM_TRY(build_timed_chunk(tc, chunk));
+    const ChunkKey key = chunk_key();
+    TimedChunk* found = cached(timed_, key);
+    if (!found) {
+      if (timed_.size() >= kMaxCachedChunks) {  // drop the oldest, once it has run
+        SLM_HIP(hipStreamSynchronize(stream));
+        free_timed(timed_.front().second);
+        timed_.erase(timed_.begin());
+      }
+      TimedChunk fresh;
+      SLM_TRY(build_timed_chunk(fresh, chunk));
+      timed_.push_back({key, std::move(fresh)});
+      found = &timed_.back().second;
+    }
+    TimedChunk& tc = *found;
     SLM_HIP(hipGraphLaunch(tc.exec, stream));
     for (uint32_t j = 0; j < chunk; j++) {  // the buffer swaps the captured steps make
       if (tc.sorted[j]) {
@@ -1821,20 +1890,20 @@ class Engine {
     return SLM_OK;
   }
 
+  static void free_timed(TimedChunk& tc) {
+    if (tc.exec) (void)hipGraphExecDestroy(tc.exec);
+    tc.exec = nullptr;
+    destroy_events(tc.ev);
+    tc.sorted.clear();
+  }
+
+  // Destroys every captured chunk.  Callers have waited for the engine's
+  // stream, which every launch of them joins back into.
   void drop_graphs() {
-    for (auto& row : graphs_)
-      for (auto& g : row)
-        if (g) {
-          (void)hipGraphExecDestroy(g);
-          g = nullptr;
-        }
-    for (auto& row : timed_)
-      for (auto& tc : row) {
-        if (tc.exec) (void)hipGraphExecDestroy(tc.exec);
-        tc.exec = nullptr;
-        destroy_events(tc.ev);
-        tc.sorted.clear();
-      }
+    for (auto& e : graphs_) (void)hipGraphExecDestroy(e.second);
+    graphs_.clear();
+    for (auto& e : timed_) free_timed(e.second);
+    timed_.clear();
   }
 };
 
@@ -2153,7 +2222,7 @@ int slm_autotune(slm_sim* s, uint32_t trial_steps, slm_tuning* out) {
 
 static int check_render(const uint8_t* lut, uint32_t tex_w, uint32_t tex_h, const void* out) {
   if (!lut || !out) return fail(SLM_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (tex_w == 0 || tex_h == 0 || tex_w > kMaxSide || tex_h > kMaxSide ||
+  if (tex_w == 0 || tex_h == 0 || tex_w > kMaxTexSide || tex_h > kMaxTexSide ||
       static_cast<uint64_t>(tex_w) * tex_h > (1ull << 30))
     return fail(SLM_ERR_INVALID_ARGUMENT, "texture must be 1..2^24 on a side, <= 2^30 pixels");
   return SLM_OK;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/slimemold.h"
 
 namespace slm {
@@ -106,6 +108,9 @@ struct Pool {
   int stochastic_rounding;     // bf16 trail-pass stores round stochastically (deterministic hash)
   int streaming;               // trail-pass rows: 1 non-temporal stores, 2 non-temporal loads
   int gaussian;                // trail pass diffuses with the Gaussian, not the 3x3 box
+  int window_sensing;          // one-agent-per-thread workgroups sense through an LDS window
+  uint32_t* claim_agents;      // persistent agent / trail passes: kClaimWords counters each
+  uint32_t* claim_trail;       // (nullptr: one workgroup per logical workgroup)
   uint32_t total_gtiles;       // Gaussian-pass tiles of the pool
 };
 
@@ -126,6 +131,79 @@ __device__ inline uint32_t xcd_logical_block(uint32_t b, uint32_t nb) {
   uint32_t per = nb >> 3, rem = nb & 7u;
   return xcd < rem ? xcd * (per + 1u) + idx : rem * (per + 1u) + (xcd - rem) * per + idx;
 }
+
+// ---- persistent launches -------------------------------------------------------
+// A persistent pass launches only the workgroups that are resident at once
+// (persistent_grid); each claims logical workgroups of the pass one at a time
+// from device counters until none are left, so every workgroup exits when its
+// claims run out.  Workgroup b claims from the contiguous range of logical
+// workgroups that xcd_logical_block gives XCD b % 8 in a one-workgroup-per-
+// block launch, so neighbouring work still shares an L2.  claim[0..7] count
+// the claims per range, claim[8] the workgroups that have finished claiming;
+// the last of those resets all nine to zero, ready for the next launch (they
+// are zeroed once when allocated), so no memset is needed between launches.
+constexpr int kClaimWords = 9;
+
+template <class Body>
+__device__ inline void persistent_blocks(uint32_t* claim, uint32_t nb, uint32_t* s_claim,
+                                         Body&& body) {
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint32_t per = nb >> 3, rem = nb & 7u;
+  const uint32_t lo = xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per;
+  const uint32_t cnt = per + (xcd < rem ? 1u : 0u);
+  for (;;) {
+    if (threadIdx.x == 0) *s_claim = atomicAdd(&claim[xcd], 1u);
+    __syncthreads();
+    const uint32_t k = *s_claim;
+    __syncthreads();
+    if (k >= cnt) break;  // uniform
+    body(lo + k);
+  }
+  if (threadIdx.x == 0 && atomicAdd(&claim[8], 1u) == gridDim.x - 1u)
+    for (int x = 0; x < kClaimWords; x++) atomicExch(&claim[x], 0u);
+}
+
+// Workgroups of `kernel` resident on the whole device at once (at least 8, a
+// multiple of 8, and no more than the pass's nb logical workgroups rounded up
+// to 8), for a persistent launch.
+// Resident workgroups per kernel (0 until measured), keyed by the kernel.
+inline int& resident_of(const void* kernel) {
+  static std::mutex mu;
+  static const void* keys[128];
+  static int vals[128];
+  static int n = 0;
+  static int overflow = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  for (int i = 0; i < n; i++)
+    if (keys[i] == kernel) return vals[i];
+  if (n == 128) return overflow = 0;
+  keys[n] = kernel;
+  vals[n] = 0;
+  return vals[n++];
+}
+
+template <class K>
+uint32_t persistent_grid(K* kernel, uint32_t nb, int block) {
+  int& resident = resident_of(reinterpret_cast<const void*>(kernel));
+  if (resident <= 0) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess)
+      per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 8;
+    (void)hipGetLastError();
+    resident = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 8);
+  }
+  uint64_t g = static_cast<uint64_t>(resident) < nb ? static_cast<uint64_t>(resident) : nb;
+  g = (g + 7u) & ~uint64_t(7u);
+  return static_cast<uint32_t>(g < 8u ? 8u : g);
+}
+
+// Fill persistent_grid's per-kernel caches outside any stream capture (the
+// engine calls these when it is created with persistent passes).
+void prime_persistent_agents();
+void prime_persistent_trail();
 
 inline uint64_t storage_bytes(int storage) { return storage == SLM_STORAGE_F32 ? 4 : 2; }
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t>((n + b - 1) / b); }

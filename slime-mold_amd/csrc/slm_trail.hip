@@ -19,6 +19,7 @@
 // adjacent lanes, and only lanes at a wave edge or at the wrap load a halo
 // cell.  Rows of other lengths use 64-cell units, one cell per lane.
 #include <cmath>
+#include <type_traits>
 
 #include "slm_internal.h"
 
@@ -110,35 +111,42 @@ struct Codec<SLM_STORAGE_F16> {
 // index and the value, so every cell and step draws differently.
 constexpr int kStorageBf16Sr = 3;
 
-__device__ inline uint32_t bf16_sr(float v, uint32_t cell) {
+// The first multiply of the draw, cell * kSrK1 (mod 2^32), is linear in the
+// cell: callers that store consecutive cells, or rows a fixed stride apart,
+// step it with one add per cell instead of a quarter-rate 32-bit multiply
+// (the hashed cell `hc` below); the bits are the same.
+constexpr uint32_t kSrK1 = 0x9e3779b1u;
+__device__ inline uint32_t bf16_sr_hashed(float v, uint32_t hc) {
   uint32_t b = __float_as_uint(v);  // v >= 0 and finite
-  uint32_t r = ((cell * 0x9e3779b1u) ^ b) * 0x85ebca6bu;
+  uint32_t r = (hc ^ b) * 0x85ebca6bu;
   return (b + (r >> 16)) >> 16;
 }
 
 template <>
 struct Codec<kStorageBf16Sr> : Codec<SLM_STORAGE_BF16> {
-  __device__ static uint32_t encode_at(float v, uint32_t cell) { return bf16_sr(v, cell); }
-  __device__ static uint4 pack_values_at(const float (&v)[8], uint32_t cell) {
-    return make_uint4(bf16_sr(v[0], cell) | (bf16_sr(v[1], cell + 1u) << 16),
-                      bf16_sr(v[2], cell + 2u) | (bf16_sr(v[3], cell + 3u) << 16),
-                      bf16_sr(v[4], cell + 4u) | (bf16_sr(v[5], cell + 5u) << 16),
-                      bf16_sr(v[6], cell + 6u) | (bf16_sr(v[7], cell + 7u) << 16));
+  __device__ static uint32_t encode_hashed(float v, uint32_t hc) { return bf16_sr_hashed(v, hc); }
+  __device__ static uint4 pack_values_hashed(const float (&v)[8], uint32_t hc) {
+    uint32_t e[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) e[k] = bf16_sr_hashed(v[k], hc + static_cast<uint32_t>(k) * kSrK1);
+    return make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16),
+                      e[6] | (e[7] << 16));
   }
 };
 
-// Stores of the trail pass, given the instance-local index of the first cell.
+// Stores of the trail pass, given the hashed instance-local index of the
+// first cell, hc = cell * kSrK1 (only stochastic rounding reads it).
 template <int S>
-__device__ inline uint4 pack_out(const float (&v)[Codec<S>::kVec], uint32_t cell) {
+__device__ inline uint4 pack_out(const float (&v)[Codec<S>::kVec], uint32_t hc) {
   if constexpr (S == kStorageBf16Sr)
-    return Codec<S>::pack_values_at(v, cell);
+    return Codec<S>::pack_values_hashed(v, hc);
   else
     return Codec<S>::pack_values(v);
 }
 template <int S>
-__device__ inline uint32_t encode_out(float v, uint32_t cell) {
+__device__ inline uint32_t encode_out(float v, uint32_t hc) {
   if constexpr (S == kStorageBf16Sr)
-    return Codec<S>::encode_at(v, cell);
+    return Codec<S>::encode_hashed(v, hc);
   else
     return Codec<S>::encode(v);
 }
@@ -298,7 +306,7 @@ __device__ inline void strip_unit(const typename Codec<S>::elem* in, typename Co
             }
             o[k] = diffuse_out<M>(cur[k + 1], sum, p);
           }
-          store_row(out + static_cast<size_t>(yy) * w + c, pack_out<S>(o, yy * w + c),
+          store_row(out + static_cast<size_t>(yy) * w + c, pack_out<S>(o, (yy * w + c) * kSrK1),
                     (streaming & kStreamStores) != 0);
         }
 #pragma unroll
@@ -369,8 +377,8 @@ __device__ inline void strip_unit_preload(const typename Codec<S>::elem* in,
           }
           o[k] = diffuse_out<M>(cur[k + 1], sum, p);
         }
-        store_row(out + static_cast<size_t>(y0 + r) * w + c, pack_out<S>(o, (y0 + r) * w + c),
-                  (streaming & kStreamStores) != 0);
+        store_row(out + static_cast<size_t>(y0 + r) * w + c,
+                  pack_out<S>(o, ((y0 + r) * w + c) * kSrK1), (streaming & kStreamStores) != 0);
       }
 #pragma unroll
       for (int k = 0; k < V + 2; k++) {
@@ -416,7 +424,7 @@ __device__ inline void cell_unit(const typename Codec<S>::elem* in, typename Cod
     }
   }
   out[i] = static_cast<typename Codec<S>::elem>(
-      encode_out<S>(diffuse_out<kMixGeneral>(centre, s, p), static_cast<uint32_t>(i)));
+      encode_out<S>(diffuse_out<kMixGeneral>(centre, s, p), static_cast<uint32_t>(i) * kSrK1));
 }
 
 struct TrailArgs {
@@ -428,15 +436,16 @@ struct TrailArgs {
   uint32_t count, total_units;
   int xcd_remap;
   int streaming;  // kStreamStores | kStreamLoads: non-temporal strip row stores / loads
+  uint32_t* claim;  // persistent launch: claim counters (nullptr: one workgroup per block)
+  uint32_t blocks;  // logical workgroups of the pass
 };
 
-// Tables as restrict arguments: their wave-uniform loads become scalar loads.
+// One logical workgroup `blk` (kTrailWavesPerBlock wave units) of the pass.
 template <int S, int R>
-__global__ __launch_bounds__(kTrailBlock) void trail_kernel(
-    TrailArgs a, const Instance* __restrict__ insts, const DevParams* __restrict__ params,
-    const uint32_t* __restrict__ block_inst) {
+__device__ inline void trail_block(const TrailArgs& a, const Instance* __restrict__ insts,
+                                   const DevParams* __restrict__ params,
+                                   const uint32_t* __restrict__ block_inst, uint32_t blk) {
   using elem = typename Codec<S>::elem;
-  uint32_t blk = a.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
   uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   uint32_t lane = threadIdx.x & 63u;
   uint32_t u = blk * kTrailWavesPerBlock + wave;
@@ -456,6 +465,22 @@ __global__ __launch_bounds__(kTrailBlock) void trail_kernel(
     strip_dispatch<S, kMixCentre, R>(in, out, inst, p, unit, lane, a.streaming);
   else
     strip_dispatch<S, kMixGeneral, R>(in, out, inst, p, unit, lane, a.streaming);
+}
+
+// Tables as restrict arguments: their wave-uniform loads become scalar loads.
+// With a.claim set the launch is persistent (persistent_blocks).
+template <int S, int R>
+__global__ __launch_bounds__(kTrailBlock) void trail_kernel(
+    TrailArgs a, const Instance* __restrict__ insts, const DevParams* __restrict__ params,
+    const uint32_t* __restrict__ block_inst) {
+  __shared__ uint32_t s_claim;
+  if (!a.claim) {
+    uint32_t blk = a.xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    trail_block<S, R>(a, insts, params, block_inst, blk);
+    return;
+  }
+  persistent_blocks(a.claim, a.blocks, &s_claim,
+                    [&](uint32_t blk) { trail_block<S, R>(a, insts, params, block_inst, blk); });
 }
 
 __global__ void set_params_kernel(DevParams* dst, DevParams src) { *dst = src; }
@@ -547,17 +572,21 @@ __global__ __launch_bounds__(256) void mass_final_kernel(const double* partials,
 
 template <int S>
 void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t stream) {
+  const uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
   TrailArgs a{in,         out,         pool.inst,      pool.params, pool.unit_block_inst,
-              pool.count, pool.total_units, pool.xcd_remap, pool.streaming};
-  uint32_t nb = blocks_for(pool.total_units, kTrailWavesPerBlock);
+              pool.count, pool.total_units, pool.xcd_remap, pool.streaming, pool.claim_trail, nb};
+  auto go = [&](auto* kernel) {
+    const uint32_t grid = a.claim ? persistent_grid(kernel, nb, kTrailBlock) : nb;
+    kernel<<<grid, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
+  };
   if (pool.preload_rows == 4)
-    trail_kernel<S, 4><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
+    go(trail_kernel<S, 4>);
   else if (pool.preload_rows == 8)
-    trail_kernel<S, 8><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
+    go(trail_kernel<S, 8>);
   else if (pool.preload_rows == 16)
-    trail_kernel<S, 16><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
+    go(trail_kernel<S, 16>);
   else
-    trail_kernel<S, 0><<<nb, kTrailBlock, 0, stream>>>(a, a.inst, a.params, a.block_inst);
+    go(trail_kernel<S, 0>);
 }
 
 
@@ -693,6 +722,9 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
     h[r] = sum;
   }
   const uint32_t x = x0 + static_cast<uint32_t>(c);
+  // hashed cell index of (x, y) for stochastic rounding, stepped a row at a time
+  const uint32_t hstep = W * kSrK1;
+  uint32_t hc = (y0 * W + x) * kSrK1;
   if (vec) {
     uint32_t o[kGaussTH];
 #pragma unroll
@@ -703,7 +735,8 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
       for (int j = 1; j <= 2 * R; j++) g = __builtin_fmaf(w[j < R ? R - j : j - R], h[r + j], g);
       float l = s_t[r + R][c + R] * p.one_minus_rate;
       float rr = g * p.rate;
-      o[r] = y < H ? encode_out<S>(l + rr, y * W + x) : 0u;
+      o[r] = y < H ? encode_out<S>(l + rr, hc) : 0u;
+      hc += hstep;
     }
     __syncthreads();  // every read of s_t is done: it becomes the output stage
     elem* s_o = reinterpret_cast<elem*>(&s_t[0][0]);  // [kGaussTH][kGaussTW]
@@ -734,8 +767,8 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
     float centre = s_t[r + R][c + R];
     float l = centre * p.one_minus_rate;
     float rr = g * p.rate;
-    const uint32_t cell = y * W + x;
-    dst[cell] = static_cast<elem>(encode_out<S>(l + rr, cell));
+    dst[y * W + x] = static_cast<elem>(encode_out<S>(l + rr, hc));
+    hc += hstep;
   }
 }
 
@@ -939,6 +972,20 @@ void launch_render(const RenderArgs& a, const LutBytes& lut, int storage, hipStr
   else
     dec ? render_kernel<SLM_STORAGE_F32, true><<<nb, 256, 0, stream>>>(a, lut)
         : render_kernel<SLM_STORAGE_F32, false><<<nb, 256, 0, stream>>>(a, lut);
+}
+
+void prime_persistent_trail() {
+  auto each = [](auto s_tag) {
+    constexpr int S = decltype(s_tag)::value;
+    (void)persistent_grid(trail_kernel<S, 0>, 8, kTrailBlock);
+    (void)persistent_grid(trail_kernel<S, 4>, 8, kTrailBlock);
+    (void)persistent_grid(trail_kernel<S, 8>, 8, kTrailBlock);
+    (void)persistent_grid(trail_kernel<S, 16>, 8, kTrailBlock);
+  };
+  each(std::integral_constant<int, SLM_STORAGE_F32>{});
+  each(std::integral_constant<int, SLM_STORAGE_BF16>{});
+  each(std::integral_constant<int, kStorageBf16Sr>{});
+  each(std::integral_constant<int, SLM_STORAGE_F16>{});
 }
 
 void launch_total_mass(const Pool& pool, const void* t, double* partials, double* out,

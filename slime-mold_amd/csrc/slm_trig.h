@@ -130,15 +130,11 @@ SLM_TRIG_FN float slm_reduce_payne_hanek(float ax, int* q_out) {
   return (float)((double)f53 * SLM_PIO2_X2M53_D);
 }
 
-/* sin and cos of one float argument (see the header comment). */
-SLM_TRIG_FN void slm_sincosf(float x, float* s_out, float* c_out) {
+/* sin and cos of a finite float argument (see the header comment): callers
+ * whose arguments are finite by construction (the agent update) skip the
+ * NaN/infinity test of slm_sincosf below; the results are the same. */
+SLM_TRIG_FN void slm_sincosf_finite(float x, float* s_out, float* c_out) {
   float ax = __builtin_fabsf(x);
-  if (!(ax <= 3.4028234663852886e38f)) { /* NaN or infinity */
-    float nan = slm_f32_from_bits(0x7fc00000u);
-    *s_out = nan;
-    *c_out = nan;
-    return;
-  }
   float r;
   int q;
   if (ax < SLM_TRIG_SMALL_MAX) {
@@ -168,6 +164,17 @@ SLM_TRIG_FN void slm_sincosf(float x, float* s_out, float* c_out) {
   if (slm_f32_bits(x) >> 31) sn = -sn;
   *s_out = sn;
   *c_out = cs;
+}
+
+/* sin and cos of one float argument; NaN for NaN or infinity. */
+SLM_TRIG_FN void slm_sincosf(float x, float* s_out, float* c_out) {
+  if (!(__builtin_fabsf(x) <= 3.4028234663852886e38f)) { /* NaN or infinity */
+    float nan = slm_f32_from_bits(0x7fc00000u);
+    *s_out = nan;
+    *c_out = nan;
+    return;
+  }
+  slm_sincosf_finite(x, s_out, c_out);
 }
 
 SLM_TRIG_FN float slm_sinf(float x) {

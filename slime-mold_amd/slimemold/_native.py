@@ -28,6 +28,8 @@ FLAG_F64_POSITIONS = 0x2
 FLAG_TILED_SENSING = 0x4
 FLAG_STOCHASTIC_ROUNDING = 0x8
 FLAG_GAUSSIAN_DIFFUSE = 0x10
+FLAG_WINDOW_SENSING = 0x20
+FLAG_PERSISTENT = 0x40
 
 
 class NativeLibraryMissing(RuntimeError):

@@ -49,6 +49,7 @@ class Simulation:
                  resort_interval: int = 0, graph_chunk: int = 100, sort_cell_log2: int = 4,
                  device: int = 0, xcd_remap: bool = True, f64_positions: bool = False,
                  tiled_sensing: bool = False, stochastic_rounding: bool = False,
+                 window_sensing: bool = False, persistent: bool = False,
                  gaussian: bool = False):
         cfg = N.default_config()
         cfg.width = width
@@ -63,6 +64,8 @@ class Simulation:
         cfg.flags = ((0 if xcd_remap else N.FLAG_NO_XCD_REMAP)
                      | (N.FLAG_F64_POSITIONS if f64_positions else 0)
                      | (N.FLAG_TILED_SENSING if tiled_sensing else 0)
+                     | (N.FLAG_WINDOW_SENSING if window_sensing else 0)
+                     | (N.FLAG_PERSISTENT if persistent else 0)
                      | (N.FLAG_STOCHASTIC_ROUNDING if stochastic_rounding else 0)
                      | (N.FLAG_GAUSSIAN_DIFFUSE if gaussian else 0))
         self._params = _params(params)

@@ -6,7 +6,7 @@ the f64-position mode must equal the oracle's f64-position restatement.
 import numpy as np
 import pytest
 
-from parity_util import PRESET_NAMES, assert_bitwise, preset
+from parity_util import PRESET_NAMES, assert_bitwise, preset, run_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -217,27 +217,61 @@ def test_instance_groups_with_params_clear_render_and_mass(slm):
         assert_bitwise(s5[k][1], s1[k][1], f"instance {k} trail")
 
 
-def test_c3_mix_sampled_instances_16_steps(slm, orc):
-    """The batched leg's own workload: the 256-instance mix in bf16 with
-    stochastic rounding, 128 instance groups, re-sort every 20 steps, 16 steps
-    in graph chunks.  Instances are independent, so a sample of them (every
-    grid size, small and large agent counts) is checked against the oracle bit
-    for bit."""
+@pytest.mark.parametrize("storage,window", [("f32", False), ("bf16sr", True)])
+def test_persistent_passes_equal_oracle_over_graph_replays(slm, orc, storage, window):
+    """Persistent agent and trail passes (resident workgroups claiming the
+    pass's work from counters the kernels reset themselves): every instance
+    equals the oracle after eager steps and several replays of the captured
+    chunks, and instance groups are refused."""
+    specs = mixed_specs(slm, seed=4)
+    sr = storage == "bf16sr"
+    print(f"[persistent] {len(specs)} instances {storage} window={window}", flush=True)
+    with slm.Batch(specs, storage="bf16" if sr else storage, stochastic_rounding=sr,
+                   resort_interval=3, graph_chunk=4, sort_cell_log2=3, persistent=True,
+                   window_sensing=window) as b:
+        starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+        b.step(3)   # eager
+        b.step(12)  # three replays of captured chunks
+        b.synchronize()
+        b.step(5)   # one more replay and an eager step
+        for k, sp in enumerate(specs):
+            a, t = starts[k]
+            orc.step(a, t, sp.params, 20, storage=storage)
+            assert_bitwise(b.read_agents(k), a, f"instance {k} agents")
+            assert_bitwise(b.read_trail(k), t, f"instance {k} trail")
+        with pytest.raises(slm.SlimeMoldError) as e:
+            b.set_tuning(instance_groups=2)
+        assert e.value.code == -5
+    s = preset(slm, "Default")
+    ga, gt, oa, ot = run_parity(slm, orc, 1024, 512, 1 << 17, 2, s, 11, resort_interval=5,
+                                sort_cell_log2=3, graph_chunk=5, persistent=True)
+    assert_bitwise(ga, oa, "single persistent agents")
+    assert_bitwise(gt, ot, "single persistent trail")
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16sr"])
+def test_c3_mix_sampled_instances_16_steps(slm, orc, storage):
+    """The batched workload: the 256-instance mix with an f32 trail (the mix
+    as configured) and in bf16 with stochastic rounding (the batched leg),
+    128 instance groups, re-sort every 20 steps, 16 steps in graph chunks.
+    Instances are independent, so a sample of them (every grid size, small and
+    large agent counts) is checked against the oracle bit for bit."""
     import slimemold.workloads as wl
     specs = wl.batch_mix(256)
     sample = sorted({0, 1, 2, 3, 64, 127, 128, 200, 255} |
                     {min(range(256), key=lambda k: specs[k].agent_count),
                      max(range(256), key=lambda k: specs[k].agent_count)})
-    print(f"[c3 parity] 256 instances, sampled {sample}", flush=True)
-    with slm.Batch(specs, storage="bf16", stochastic_rounding=True, resort_interval=20,
-                   graph_chunk=4, sort_cell_log2=3) as b:
+    print(f"[c3 parity] 256 instances {storage}, sampled {sample}", flush=True)
+    sr = storage == "bf16sr"
+    with slm.Batch(specs, storage="bf16" if sr else storage, stochastic_rounding=sr,
+                   resort_interval=20, graph_chunk=4, sort_cell_log2=3) as b:
         b.set_tuning(trail_rows=8, instance_groups=128)
         starts = {k: (b.read_agents(k), b.read_trail(k)) for k in sample}
         b.step(16)
         got = {k: (b.read_agents(k), b.read_trail(k)) for k in sample}
     for k in sample:
         a, t = starts[k]
-        orc.step(a, t, specs[k].params, 16, storage="bf16sr")
+        orc.step(a, t, specs[k].params, 16, storage=storage)
         assert_bitwise(got[k][0], a, f"instance {k} agents")
         assert_bitwise(got[k][1], t, f"instance {k} trail")
 

@@ -92,6 +92,67 @@ def test_resort_gives_identical_results(slm, orc, interval, tiled):
     assert_bitwise(gt, ot, "trail")
 
 
+WINDOW_CASES = [
+    # (w, h, n, storage, preset, resort, bin log2, steps): ragged and tiny grids,
+    # rows of whole 16-byte chunks only (others sense globally), every storage,
+    # sensor distances from 20 to 225 (windows too large fall back), no re-sort
+    dict(w=512, h=512, n=1 << 16, storage="f32", name="Firecracker Trees", resort=1, cl=3, steps=20),
+    dict(w=1024, h=256, n=60000, storage="f32", name="Default", resort=7, cl=3, steps=24),
+    dict(w=520, h=300, n=40000, storage="bf16", name="Waves", resort=2, cl=4, steps=16),
+    dict(w=256, h=200, n=30000, storage="f16", name="Sponge", resort=5, cl=3, steps=16),
+    dict(w=600, h=600, n=50000, storage="f32", name="Snake", resort=3, cl=3, steps=12),
+    dict(w=96, h=17, n=3000, storage="bf16", name="Mesh", resort=1, cl=2, steps=12),
+    dict(w=8, h=8, n=500, storage="f32", name="Curls", resort=1, cl=2, steps=12),
+    dict(w=333, h=129, n=20000, storage="f32", name="Threads", resort=4, cl=3, steps=12),
+    dict(w=400, h=300, n=25000, storage="f32", name="Default", resort=0, cl=4, steps=12),
+]
+
+
+@pytest.mark.parametrize("case", WINDOW_CASES,
+                         ids=lambda c: f"{c['w']}x{c['h']}-{c['storage']}-{c['name'].replace(' ', '')}")
+def test_window_sensing_gives_identical_results(slm, orc, case):
+    """Windowed sensing stages each workgroup's bounding-box window of the
+    trail in LDS; taps outside it and workgroups whose window does not fit
+    read global memory, so the result must be the oracle's bit for bit."""
+    s = preset(slm, case["name"])
+    ga, gt, oa, ot = run_parity(slm, orc, case["w"], case["h"], case["n"], 31, s, case["steps"],
+                                storage=case["storage"], resort_interval=case["resort"],
+                                sort_cell_log2=case["cl"], graph_chunk=4, window_sensing=True)
+    assert_bitwise(ga, oa, "agents")
+    assert_bitwise(gt, ot, "trail")
+
+
+def test_window_sensing_with_f64_positions_and_groups(slm, orc):
+    """The window's bounding box from f64 positions, and windowed sensing in a
+    batch with instance groups (each group's launch offsets its tables)."""
+    s = preset(slm, "Default")
+    ga, gt, oa, ot = run_parity(slm, orc, 640, 480, 40000, 5, s, 10, resort_interval=3,
+                                sort_cell_log2=3, graph_chunk=0, window_sensing=True,
+                                f64_positions=True)
+    # f64 positions: the oracle's f32 path equals the f64 one where positions
+    # stay exact; compare against the f64-position restatement instead
+    with slm.Simulation(640, 480, 40000, seed=5, params=s, resort_interval=3,
+                        sort_cell_log2=3, graph_chunk=0, f64_positions=True) as ref:
+        ref.step(10)
+        assert_bitwise(ga, ref.read_agents(), "f64 agents vs global sensing")
+        assert_bitwise(gt, ref.read_trail(), "f64 trail vs global sensing")
+    specs = [slm.InstanceSpec(w, h, n, seed=k, params=preset(slm, nm))
+             for k, (w, h, n, nm) in enumerate([(256, 256, 20000, "Default"),
+                                                (512, 128, 30000, "Waves"),
+                                                (128, 128, 5000, "Snake"),
+                                                (64, 64, 700, "Sponge")])]
+    with slm.Batch(specs, resort_interval=2, sort_cell_log2=3, graph_chunk=3) as b:
+        b.set_tuning(instance_groups=3, tiled_sensing=2)
+        assert b.get_tuning().tiled_sensing == 2
+        starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+        b.step(9)
+        for k, sp in enumerate(specs):
+            a, t = starts[k]
+            orc.step(a, t, sp.params, 9)
+            assert_bitwise(b.read_agents(k), a, f"instance {k} agents")
+            assert_bitwise(b.read_trail(k), t, f"instance {k} trail")
+
+
 def test_graphs_match_eager(slm):
     s = preset(slm, "Snake")
     outs = []

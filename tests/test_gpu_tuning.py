@@ -50,7 +50,7 @@ def test_set_tuning_rejects_bad_values(slm):
         before = sim.get_tuning().as_dict()
         for fields in [dict(agents_per_group=300), dict(agents_per_group=8192),
                        dict(agents_per_group=128), dict(trail_rows=3), dict(trail_rows=128),
-                       dict(sort_cell_log2=13), dict(tiled_sensing=2),
+                       dict(sort_cell_log2=13), dict(tiled_sensing=3),
                        dict(instance_groups=4097), dict(streaming=4)]:
             with pytest.raises(slm.SlimeMoldError) as e:
                 sim.set_tuning(**fields)
@@ -139,8 +139,31 @@ def test_step_timed_matches_step(slm, storage):
             print(f"[timed] {tm.agents_ms:.3f} {tm.trail_ms:.3f} {tm.sort_ms:.3f} "
                   f"{tm.agents_launches} {tm.trail_launches} {tm.sort_launches}", flush=True)
         assert t.agents_launches == 35 and t.trail_launches == 35 and t2.trail_launches == 20
-        assert t.sort_launches >= 11 and t2.sort_launches >= 6
+        # re-sorts on absolute steps 0, 3, 6, ...: 12 in steps 0..34, 7 in 35..54
+        assert t.sort_launches == 12 and t2.sort_launches == 7
         assert t.agents_ms > 0 and t.trail_ms > 0 and t.sort_ms > 0
         assert t2.agents_ms > 0 and t2.trail_ms > 0
         # per launch, times are those of real kernels (not stale or zero events)
         assert 0.0005 < t.trail_ms / t.trail_launches < 5.0
+
+
+def test_resort_schedule_is_absolute_across_call_sizes(slm):
+    """Re-sorts fall on steps 0, k, 2k, ... of the simulation whatever the
+    calls' sizes and whether steps run eagerly or from captured chunks (a
+    chunk that k does not divide is captured per starting phase)."""
+    s = preset(slm, "Default")
+    kw = dict(seed=9, params=s, graph_chunk=20, resort_interval=15)
+    with slm.Simulation(256, 128, 9000, **kw) as a, slm.Simulation(256, 128, 9000, **kw) as b:
+        total = 0
+        for n in (7, 100, 13, 41, 60):  # eager remainders and chunks at shifted phases
+            tm = b.step_timed(n)
+            want = sum(1 for i in range(total, total + n) if i % 15 == 0)
+            print(f"[schedule] steps {total}..{total + n - 1}: {tm.sort_launches} re-sorts "
+                  f"(want {want})", flush=True)
+            assert tm.sort_launches == want and tm.agents_launches == n
+            total += n
+        for n in (7, 100, 13, 41, 60):
+            a.step(n)  # the captured-graph path of slm_step, same schedule
+        a.synchronize()
+        assert_bitwise(b.read_agents(), a.read_agents(), "agents")
+        assert_bitwise(b.read_trail(), a.read_trail(), "trail")
