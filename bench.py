@@ -8,10 +8,12 @@ whole grid.
 
 Batched leg: the 256-instance mix (grids 256^2..2048^2, 2^14..2^22 agents,
 parameters swept over the preset range) with a bf16 trail, all instances
-advanced together (autotuned into instance groups on four streams).  Separate
-legs: the same instances as 256 separate f32 simulations with the batch's
-tuning - the plain path on the batched workload - stepped one simulation at a
-time ("batched_vs_separate_serial") and all enqueued at once
+advanced together (autotuned into instance groups on four streams).  Beside
+it the same batch with an f32 trail ("batched_f32", the mix as configured:
+the plain FP32 path on the batched workload, in one pool;
+"batched_vs_batched_f32").  Separate legs: the same instances as 256 separate
+f32 simulations with the batch's tuning, stepped one simulation at a time
+("batched_vs_separate_serial") and all enqueued at once
 ("batched_vs_separate_concurrent").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
@@ -48,18 +50,34 @@ METRIC = "agent-updates/s + diffuse GB/s (% of HBM peak), plain FP32 vs batched 
 HBM_PEAK_GBPS = 8000.0
 STORAGE_BYTES = {"f32": 4, "bf16": 2, "f16": 2}
 
-# Counter calibration measured with --copy-probe-mib 1024 under rocprofv3 on
-# MI355X: FETCH_SIZE reads half the bytes a plain copy moves, WRITE_SIZE all.
+# Counter calibration (tests/counter_calibration.py under rocprofv3 on MI355X:
+# a device-to-device copy of known size): FETCH_SIZE reads half the bytes a
+# plain copy moves, WRITE_SIZE all of them.
 FETCH_SIZE_SCALE = 2.0
 WRITE_SIZE_SCALE = 1.0
 
+# HBM bytes per launch of the plain leg's dominant kernel, measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch, averaged over
+# the dispatches of the timed steps, scaled as above) for exactly the run a
+# line describes: keyed by the workload, the launch tuning (re-sort interval,
+# bin log2, trail rows, agents per group, streaming) and the timed and warmup
+# step counts.  A line whose run has no entry reports traffic null.  Sources:
+# profiles/agents_kernel_plain_traffic.md.
+MEASURED_TRAFFIC = {
+    # (kernel, W, H, N, storage, tuning, steps, warmup): (FETCH KiB, WRITE KiB)
+    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3), 2000, 100):
+        (168985, 218185),
+}
 
-# HBM bytes per launch of the plain leg's dominant kernel at the default
-# workload (4096^2 f32, 2^24 agents, re-sort every 15 steps into 8x8 bins,
-# 16-row trail tiles, streaming 3): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x
-# 1.0 per dispatch averaged over the 2000 timed steps, recorded in
-# profiles/agents_trail_c1_counters.md.
-MEASURED_TRAFFIC = {("agents_kernel", 4096, 4096, 1 << 24, "f32"): (168985 * 2 + 218185) * 1024}
+
+def measured_traffic(kernel, args):
+    key = (kernel, args.width, args.height, 1 << args.agents_log2, args.storage,
+           (args.resort, args.sort_cell_log2, args.trail_rows, args.agents_per_group,
+            args.streaming), args.steps, args.warmup)
+    kib = MEASURED_TRAFFIC.get(key)
+    if kib is None or args.tiled or args.window or args.autotune_plain:
+        return None
+    return int((kib[0] * FETCH_SIZE_SCALE + kib[1] * WRITE_SIZE_SCALE) * 1024)
 
 
 def agent_bytes(sb, pos64=False):
@@ -91,7 +109,17 @@ def parse():
                     help="batched/combined legs' re-sort interval before autotuning")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--sort-cell-log2", type=int, default=3, help="re-sort bin edge 2^k cells")
-    ap.add_argument("--tiled", action="store_true", help="LDS-staged tiled sensing")
+    ap.add_argument("--tiled", action="store_true",
+                    help="plain leg: LDS-staged sensing from per-tile work lists")
+    ap.add_argument("--window", action="store_true",
+                    help="plain leg: LDS-staged sensing through per-workgroup windows")
+    ap.add_argument("--batch-tuning", default="",
+                    help="batched / combined legs: a fixed tuning instead of the autotuner, as "
+                         "comma-separated field=value pairs of slm_tuning (e.g. "
+                         "instance_groups=128,resort_interval=20,sort_cell_log2=3,trail_rows=8)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="batched / combined legs: persistent agent and trail passes (one "
+                         "launch each per step over every instance; no instance groups)")
     ap.add_argument("--trail-rows", type=int, default=16,
                     help="plain leg: rows per trail-pass tile (0: chosen from the grid)")
     ap.add_argument("--agents-per-group", type=int, default=256)
@@ -111,6 +139,9 @@ def parse():
                     help="steps of each Gaussian-diffuse leg, same maps (0: skip)")
     ap.add_argument("--plain-gaussian-steps", type=int, default=2000,
                     help="steps of the plain workload with the Gaussian diffuse (0: skip)")
+    ap.add_argument("--batch-f32-steps", type=int, default=-1,
+                    help="steps of the batched workload with an f32 trail, autotuned the same "
+                         "way (-1: as many as the batched leg, 0: skip)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--separate-steps", type=int, default=300,
                     help="steps of the batched leg's instances run as separate f32 "
@@ -124,29 +155,11 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-single-thread", type=int, default=1,
                     help="also time one whole step of the CPU baseline on one thread (0: skip)")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="only copy a buffer of this many MiB device to device three times "
-                         "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes)")
     return ap.parse_args()
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def copy_probe(mib):
-    """A plain device-to-device copy of known size for counter calibration."""
-    import torch
-
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
-                      "copies": 3}))
-    return 0
 
 
 def cpu_model():
@@ -163,18 +176,22 @@ def cpu_model():
     return f"{name}, {len(os.sched_getaffinity(0))} CPUs available"
 
 
-def cpu_baseline(orc, agents, trail, settings, seconds, threads):
-    """Oracle (C, OpenMP) on the same state: one step to size the sample, then
-    as many whole steps as fit in `seconds`."""
+def cpu_baseline(orc, agents, trail, presets, seconds, threads):
+    """Oracle (C, OpenMP) on the same state and the same preset sequence as
+    the GPU leg: one step to size the sample, then as many whole steps as fit
+    in `seconds`, split in thirds over the presets in order."""
     a, t = agents.copy(), trail.copy()
     t0 = time.perf_counter()
-    orc.step(a, t, settings, 1, nthreads=threads)
+    orc.step(a, t, presets[0], 1, nthreads=threads)
     one = time.perf_counter() - t0
-    n = max(1, min(200, int(seconds / max(one, 1e-6))))
+    n = max(3, min(300, int(seconds / max(one, 1e-6))))
+    seg = [n // 3, n // 3, n - 2 * (n // 3)]
+    a, t = agents.copy(), trail.copy()
     t0 = time.perf_counter()
-    orc.step(a, t, settings, n, nthreads=threads)
+    for p, k in zip(presets, seg):
+        orc.step(a, t, p, k, nthreads=threads)
     dt = time.perf_counter() - t0
-    return n, dt
+    return n, seg, dt
 
 
 def kernel_times(handle, presets, segments, set_params):
@@ -198,10 +215,32 @@ def kernel_times(handle, presets, segments, set_params):
 
 
 def roofline(name, bytes_per_launch, seconds):
+    """Algorithmic bytes per launch over the average launch time, against the
+    HBM peak (nothing in the step is a dense contraction, so HBM is the
+    roofline every kernel here is priced against)."""
     gbps = bytes_per_launch / seconds / 1e9
     return {"kernel": name, "bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4), "traffic": None,
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(seconds * 1e6, 2)}
+
+
+# What the counters show limits each kernel (rocprofv3 TA_BUSY, SQ_INSTS_VALU,
+# SQ_BUSY_CYCLES, FETCH_SIZE / WRITE_SIZE; profiles/agents_trail_c1_counters.md).
+LIMITER = {
+    "agents_kernel": ("texture-address (TA) processing of the divergent sensor gathers, then "
+                      "VALU; DRAM traffic is well below the algorithmic bytes because the taps "
+                      "hit L2, so the kernel is not HBM-bound"),
+    "trail_kernel": "HBM bandwidth",
+}
+
+
+def graph_split(segments, chunk):
+    """(graph steps, eager steps) of slm_step over these segments: whole
+    chunks replay captured graphs, the rest of a segment launches eagerly."""
+    if chunk <= 0:
+        return 0, sum(segments)
+    g = sum(n // chunk * chunk for n in segments)
+    return g, sum(segments) - g
 
 
 def plain_leg(slm, args):
@@ -212,7 +251,8 @@ def plain_leg(slm, args):
     seg = [K // 3, K // 3, K - 2 * (K // 3)]
     sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
                          resort_interval=args.resort, graph_chunk=args.graph_chunk,
-                         sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled)
+                         sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled,
+                         window_sensing=args.window)
     sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
                    streaming=args.streaming)
     log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
@@ -323,16 +363,22 @@ def diffuse_leg(slm, side, storage, steps, graph_chunk, tune_steps, rounding="ne
 
 
 def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2, label,
-              tiled=False, tune_steps=0, rounding="nearest"):
+              tiled=False, tune_steps=0, rounding="nearest", persistent=False):
     sb = STORAGE_BYTES[storage]
-    log(f"{label}: {len(specs)} instances storage={storage} creating")
+    log(f"{label}: {len(specs)} instances storage={storage} persistent={persistent} creating")
     sr = storage == "bf16" and rounding == "stochastic"
     b = slm.Batch(specs, storage=storage, resort_interval=resort, graph_chunk=graph_chunk,
-                  sort_cell_log2=cell_log2, tiled_sensing=tiled, stochastic_rounding=sr)
+                  sort_cell_log2=cell_log2, tiled_sensing=tiled, stochastic_rounding=sr,
+                  persistent=persistent)
     info = b.info()
     log(f"{label}: {info.total_agents} agents, {info.total_cells} cells, "
         f"{info.device_bytes / 2**30:.1f} GiB; warmup {warmup}")
-    tuning = autotune(b, tune_steps, label)
+    if FIXED_TUNING:
+        b.set_tuning(**FIXED_TUNING)
+        tuning = dict(b.get_tuning().as_dict(), autotuned=False)
+        log(f"{label}: fixed tuning {tuning}")
+    else:
+        tuning = autotune(b, tune_steps, label)
     b.step(warmup)
     b.synchronize()
     t0 = time.perf_counter()
@@ -359,6 +405,7 @@ def batch_leg(slm, specs, storage, steps, warmup, resort, graph_chunk, cell_log2
         "agents_gbps": round(agent_bytes(sb) * n_agents / a_s / 1e9, 1),
         "graph_chunk": graph_chunk, "tuning": tuning, "measure_steps": M,
         "rounding": "stochastic" if sr else "nearest",
+        "persistent": persistent,
         "total_mass_sum": float(masses.sum()),
     }
 
@@ -412,10 +459,14 @@ def separate_leg(slm, specs, storage, steps, warmup, graph_chunk, tuning, serial
             "tuning": {f: tuning[f] for f in fields}}
 
 
+FIXED_TUNING = {}
+
+
 def main():
     args = parse()
-    if args.copy_probe_mib:
-        return copy_probe(args.copy_probe_mib)
+    for pair in filter(None, args.batch_tuning.split(",")):
+        k, v = pair.split("=")
+        FIXED_TUNING[k.strip()] = int(v)
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs one simulation per GPU on one GPU; "
                                    "--gpus must be 1",
@@ -434,7 +485,7 @@ def main():
         specs = slm.workloads.combined()
         res = batch_leg(slm, specs, "bf16", args.steps, args.warmup, args.batch_resort,
                         args.graph_chunk, args.sort_cell_log2, "combined", args.tiled,
-                        args.autotune, args.rounding)
+                        args.autotune, args.rounding, args.persistent)
         out = {
             "metric": METRIC, "value": res["agent_updates_per_s"], "unit": "agent-updates/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -451,11 +502,15 @@ def main():
     W, H, N = args.width, args.height, 1 << args.agents_log2
     dominant = plain["agents_roof"] if plain["agents_s"] >= plain["trail_s"] else plain["trail_roof"]
     dominant = dict(dominant)
-    dominant["traffic"] = MEASURED_TRAFFIC.get(
-        (dominant["kernel"], W, H, N, args.storage)
-        if (args.resort, args.sort_cell_log2, args.trail_rows, args.agents_per_group,
-            args.streaming) == (15, 3, 16, 256, 3)
-        else None)
+    dominant["limiter"] = LIMITER[dominant["kernel"]]
+    traffic = measured_traffic(dominant["kernel"], args)
+    dominant["traffic"] = traffic
+    if traffic is not None:
+        dram = traffic / (dominant["avg_launch_us"] * 1e-6) / 1e9
+        dominant["dram_gbps"] = round(dram, 1)
+        dominant["dram_frac"] = round(dram / HBM_PEAK_GBPS, 4)
+        dominant["traffic_source"] = "profiles/agents_kernel_plain_traffic.md (same command)"
+    graph_steps, eager_steps = graph_split(plain["segments"], args.graph_chunk)
     out = {
         "metric": METRIC,
         "value": plain["value"],
@@ -473,9 +528,12 @@ def main():
             "workload": (f"plain {args.storage} trail {W}x{H}, 2^{args.agents_log2} agents, "
                          f"presets {' -> '.join(slm.workloads.PLAIN_PRESETS)} back to back "
                          f"({'/'.join(map(str, plain['segments']))} steps), seed {args.seed}, "
-                         f"hipGraph {args.graph_chunk}-step chunks"),
+                         f"{graph_steps} of the {plain['steps']} timed steps replayed from "
+                         f"captured {args.graph_chunk}-step hipGraphs, {eager_steps} launched "
+                         f"eagerly (a segment's steps beyond its last whole chunk)"),
             "width": W, "height": H, "agents": N, "storage": args.storage, "seed": args.seed,
             "presets": slm.workloads.PLAIN_PRESETS, "graph_chunk": args.graph_chunk,
+            "graph_steps": graph_steps, "eager_steps": eager_steps,
             "tuning": plain["tuning"],
         },
         "roofline": dominant,
@@ -523,9 +581,17 @@ def main():
         specs = slm.workloads.batch_mix(args.batch_count)
         bres = batch_leg(slm, specs, args.batch_storage, args.batch_steps, 20, args.batch_resort,
                          args.graph_chunk, args.sort_cell_log2, "batched", args.tiled,
-                         args.autotune, args.rounding)
+                         args.autotune, args.rounding, args.persistent)
         out["batched"] = bres
         out["batched_vs_plain"] = round(bres["agent_updates_per_s"] / plain["value"], 3)
+        f32_steps = args.batch_steps if args.batch_f32_steps < 0 else args.batch_f32_steps
+        if f32_steps > 0:
+            fres = batch_leg(slm, specs, "f32", f32_steps, 20, args.batch_resort,
+                             args.graph_chunk, args.sort_cell_log2, "batched_f32", args.tiled,
+                             args.autotune, persistent=args.persistent)
+            out["batched_f32"] = fres
+            out["batched_vs_batched_f32"] = round(
+                bres["agent_updates_per_s"] / fres["agent_updates_per_s"], 3)
         sep_steps = args.batch_steps if args.separate_steps < 0 else args.separate_steps
         if sep_steps > 0:
             out["separate"] = {}
@@ -540,14 +606,16 @@ def main():
 
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         a0, t0 = plain["state0"]
-        n, dt = cpu_baseline(oracle, a0, t0, plain["presets"][0], args.cpu_seconds, threads)
+        n, cseg, dt = cpu_baseline(oracle, a0, t0, plain["presets"], args.cpu_seconds, threads)
         out["cpu_baseline"] = {
             "value": N * n / dt,
             "unit": "agent-updates/s",
             "cores": threads,
             "kind": "port",
             "sample": (f"C oracle (OpenMP, {threads} threads) on the plain leg's {W}x{H}, "
-                       f"{N}-agent initial state, Default preset, {n} whole steps in {dt:.1f} s"),
+                       f"{N}-agent initial state, presets "
+                       f"{' -> '.join(slm.workloads.PLAIN_PRESETS)} for "
+                       f"{'/'.join(map(str, cseg))} whole steps ({n} in {dt:.1f} s)"),
             "host": cpu_model(),
         }
         if args.cpu_single_thread:
