@@ -65,6 +65,8 @@ WRITE_SIZE_SCALE = 1.0
 # profiles/agents_kernel_plain_traffic.md.
 MEASURED_TRAFFIC = {
     # (kernel, W, H, N, storage, tuning, steps, warmup): (FETCH KiB, WRITE KiB)
+    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3), 20, 5):
+        (183378, 272155),
     ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3), 2000, 100):
         (168985, 218185),
 }

@@ -11,7 +11,8 @@ expectation.
 
   python tests/statistics_report.py --side gpu  --out gpu.json     (MI355X)
   python tests/statistics_report.py --side libm --out libm.json    (CPU oracle)
-  python tests/statistics_report.py --compare gpu.json libm.json   (markdown)
+  python tests/statistics_report.py --compare gpu.json libm.json   (markdown;
+        each side may list several files, comma-separated, whose runs are pooled)
 
 Measurement infrastructure, not a test (pytest does not collect this file).
 """
@@ -111,7 +112,6 @@ def compare(gpu, libm):
         spread = float(np.std(gm, ddof=1) / np.mean(gm)) if len(gm) > 1 else 0.0
         cells = []
         ok = abs(m) <= 0.01 or abs(m) <= 2 * se  # 1%, or not distinguishable from 0
-        verdicts = ["mass" if abs(m) <= 0.01 else None]
         for i, (lo, hi) in enumerate(OCTAVES):
             r, s = rel_diff([x["octaves"][i] for x in g], [x["octaves"][i] for x in l])
             cells.append(f"{100 * r:+.2f}% ± {100 * s:.2f}%")
@@ -127,7 +127,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--side", choices=["gpu", "libm"])
     ap.add_argument("--out")
-    ap.add_argument("--compare", nargs=2, metavar=("GPU_JSON", "LIBM_JSON"))
+    ap.add_argument("--compare", nargs=2, metavar=("GPU_JSONS", "LIBM_JSONS"))
     ap.add_argument("--seeds", type=int, default=8)
     ap.add_argument("--first-seed", type=int, default=100)
     ap.add_argument("--presets", default="Default,Sponge,Firecracker Trees,Threads,Curls,Waves,"
@@ -135,9 +135,13 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     args = ap.parse_args()
     if args.compare:
-        gpu = json.load(open(args.compare[0]))
-        libm = json.load(open(args.compare[1]))
-        print(compare(gpu["runs"], libm["runs"]))
+        def merged(paths):
+            runs = {}
+            for path in paths.split(","):
+                for name, rows in json.load(open(path))["runs"].items():
+                    runs.setdefault(name, []).extend(rows)
+            return runs
+        print(compare(merged(args.compare[0]), merged(args.compare[1])))
         return 0
     presets = [p for p in args.presets.split(",") if p]
     seeds = list(range(args.first_seed, args.first_seed + args.seeds))
