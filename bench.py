@@ -27,8 +27,9 @@ state, untimed; each leg reports its tuning under "tuning".
 
 Prints one JSON line: value = plain agent-updates/s over the timed region; the
 roofline of the plain leg's dominant kernel, whose average launch time comes
-from HIP events recorded inside the captured graph chunks of a further
-segment of as many steps, same presets, run right after the timed region; the
+from HIP events around every kernel over the same steps replayed in a second
+simulation from the same seed, tuning and warmup (the step is deterministic,
+so each launch meets the state it met in the timed region); the
 diffuse kernel's achieved bandwidth; the batched leg; and the CPU oracle timed
 on a bounded sample of the plain leg's state on the host cores.  Beside them,
 the library's Gaussian-diffuse extension at the presets' blur radius 2 (the
@@ -62,13 +63,14 @@ WRITE_SIZE_SCALE = 1.0
 # line describes: keyed by the workload, the launch tuning (re-sort interval,
 # bin log2, trail rows, agents per group, streaming) and the timed and warmup
 # step counts.  A line whose run has no entry reports traffic null.  Sources:
-# profiles/agents_kernel_plain_traffic.md.
+# profiles/agents_kernel_plain_traffic.md (20 timed steps after 5 warmup, and
+# 2000 after 100).
 MEASURED_TRAFFIC = {
     # (kernel, W, H, N, storage, tuning, steps, warmup): (FETCH KiB, WRITE KiB)
     ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3), 20, 5):
         (183378, 272155),
     ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3), 2000, 100):
-        (168985, 218185),
+        (169008, 218199),
 }
 
 
@@ -131,9 +133,6 @@ def parse():
                     help="autotune trial steps before the batched/combined leg's warmup (0: off)")
     ap.add_argument("--autotune-plain", action="store_true",
                     help="also autotune the plain leg (otherwise it runs the flags' tuning)")
-    ap.add_argument("--measure-steps", type=int, default=0,
-                    help="steps of the per-kernel timing segment after the timed region "
-                         "(0: as many as the timed region)")
     ap.add_argument("--batch-steps", type=int, default=1000, help="batched leg steps (0: skip)")
     ap.add_argument("--diffuse-steps", type=int, default=500,
                     help="steps of each diffuse-only leg (8192^2 f32, 16384^2 bf16; 0: skip)")
@@ -251,12 +250,17 @@ def plain_leg(slm, args):
     sb = STORAGE_BYTES[args.storage]
     K = args.steps
     seg = [K // 3, K // 3, K - 2 * (K // 3)]
-    sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
-                         resort_interval=args.resort, graph_chunk=args.graph_chunk,
-                         sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled,
-                         window_sensing=args.window)
-    sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
-                   streaming=args.streaming)
+
+    def make():
+        sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
+                             resort_interval=args.resort, graph_chunk=args.graph_chunk,
+                             sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled,
+                             window_sensing=args.window)
+        sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
+                       streaming=args.streaming)
+        return sim
+
+    sim = make()
     log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
         f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}")
     state0 = (sim.read_agents(), sim.read_trail()) if args.cpu_seconds > 0 else None
@@ -270,23 +274,35 @@ def plain_leg(slm, args):
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"plain: timed {K} steps in {elapsed:.3f} s")
-    M = args.measure_steps or K
-    mseg = [M // 3, M // 3, M - 2 * (M // 3)]
-    t0 = time.perf_counter()
-    a_s, t_s, s_s = kernel_times(sim, presets, mseg, sim.set_params)
-    m_elapsed = time.perf_counter() - t0
-    log(f"plain: measured {M} steps with per-kernel events in {m_elapsed:.3f} s")
     mass = sim.total_mass()
     sim.close()
+    # The kernels' own times: the same K steps again, from the same seed,
+    # tuning and warmup (the step is deterministic, so every launch sees the
+    # state it saw in the timed region), with HIP events around each kernel
+    # on the library's stream.
+    rep = make()
+    if tuning.get("autotuned"):
+        rep.set_tuning(**{f: tuning[f] for f in TUNING_FIELDS})
+    rep.step(args.warmup)
+    rep.synchronize()
+    t0 = time.perf_counter()
+    a_s, t_s, s_s = kernel_times(rep, presets, seg, rep.set_params)
+    m_elapsed = time.perf_counter() - t0
+    rep.close()
+    log(f"plain: the same {K} steps again with per-kernel events in {m_elapsed:.3f} s")
     a_bytes, t_bytes = agent_bytes(sb) * N, trail_bytes(sb) * W * H
     return {
         "value": N * K / elapsed, "elapsed": elapsed, "steps": K, "segments": seg,
-        "measure_steps": M, "measure_value": N * M / m_elapsed,
+        "measure_value": N * K / m_elapsed,
         "agents_s": a_s, "trail_s": t_s, "sort_s": s_s, "mass": mass,
         "agents_roof": roofline("agents_kernel", a_bytes, a_s),
         "trail_roof": roofline("trail_kernel", t_bytes, t_s),
         "cells": W * H, "agents": N, "state0": state0, "presets": presets, "tuning": tuning,
     }
+
+
+TUNING_FIELDS = ("resort_interval", "sort_cell_log2", "agents_per_group", "trail_rows",
+                 "tiled_sensing", "instance_groups", "streaming")
 
 
 def plain_gaussian_leg(slm, args, steps):
@@ -558,9 +574,11 @@ def main():
             "bytes_per_launch": plain["agents_roof"]["bytes_per_launch"],
         },
         "sort_avg_launch_us": round(plain["sort_s"] * 1e6, 2),
-        "kernel_timing": (f"HIP events inside the captured graph chunks of a further "
-                          f"{plain['measure_steps']} steps (same presets, thirds) right after the "
-                          f"timed region; those steps ran at "
+        "kernel_timing": (f"HIP events around every kernel on the library's stream, over the "
+                          f"timed region's {plain['steps']} steps replayed from the same seed, "
+                          f"tuning and warmup in a second simulation (the same launches on the "
+                          f"same states; event records inside the graph chunks, eager steps "
+                          f"each with their own); the replay ran at "
                           f"{plain['measure_value']:.4g} agent-updates/s"),
         "cell_updates_per_s": plain["cells"] * plain["steps"] / plain["elapsed"],
         "final_total_mass": plain["mass"],
