@@ -300,21 +300,29 @@ def test_c4_big_instance_two_steps(slm, orc):
         assert_bitwise(got[k][1], t, f"instance {k} trail")
 
 
-@pytest.mark.parametrize("storage", ["bf16", "f16"])
-def test_c2_shape_with_f64_positions_4_steps(slm, orc, storage):
+@pytest.mark.parametrize("storage,positions", [("bf16", "f64"), ("f16", "f64"),
+                                               ("bf16", "f32"), ("f16", "f32")])
+def test_c2_shape_4_steps(slm, orc, storage, positions):
     """The precision configuration itself: 8192^2 trail in bf16 or f16, 2^26
-    agents, seed 7, Default preset, f64 positions (the mode meant for grids of
-    8192 and larger), re-sorting on; 4 steps bit for bit against the oracle's
+    agents, seed 7, Default preset, re-sorting on, with f32 positions (as the
+    configuration states) and with f64 positions (the mode meant for grids of
+    8192 and larger); 4 steps bit for bit against the oracle's f32 or
     f64-position restatement."""
     s = preset(slm, "Default")
-    print(f"[c2 parity] 8192^2 n=2^26 {storage} f64 positions", flush=True)
+    p64 = positions == "f64"
+    print(f"[c2 parity] 8192^2 n=2^26 {storage} {positions} positions", flush=True)
     with slm.Simulation(8192, 8192, 1 << 26, seed=7, params=s, storage=storage,
-                        f64_positions=True, resort_interval=2, graph_chunk=2) as sim:
+                        f64_positions=p64, resort_interval=2, graph_chunk=2) as sim:
         a0 = sim.read_agents()
         t = sim.read_trail()
         sim.step(4)
         ga, gt = sim.read_agents(), sim.read_trail()
-    a = a0.astype(np.float64)
-    orc.step_p64(a, t, s, 4, storage=storage)
-    assert_bitwise(ga, a.astype(np.float32), "f64-position agents (rounded to f32)")
+    if p64:
+        a = a0.astype(np.float64)
+        orc.step_p64(a, t, s, 4, storage=storage)
+        assert_bitwise(ga, a.astype(np.float32), "f64-position agents (rounded to f32)")
+    else:
+        a = a0.copy()
+        orc.step(a, t, s, 4, storage=storage)
+        assert_bitwise(ga, a, "f32-position agents")
     assert_bitwise(gt, t, "trail")
