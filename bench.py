@@ -61,15 +61,15 @@ WRITE_SIZE_SCALE = 1.0
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch, averaged over
 # the dispatches of the timed steps, scaled as above) for exactly the run a
 # line describes: keyed by the workload, the launch tuning (re-sort interval,
-# bin log2, trail rows, agents per group, streaming) and the timed and warmup
+# bin log2, trail rows, agents per group, streaming, re-sort key) and the timed and warmup
 # step counts.  A line whose run has no entry reports traffic null.  Sources:
 # profiles/agents_kernel_plain_traffic.md (20 timed steps after 5 warmup, and
 # 2000 after 100).
 MEASURED_TRAFFIC = {
     # (kernel, W, H, N, storage, tuning, steps, warmup): (FETCH KiB, WRITE KiB)
-    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3), 20, 5):
+    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3, 0), 20, 5):
         (183378, 272155),
-    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3), 2000, 100):
+    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3, 0), 2000, 100):
         (169008, 218199),
 }
 
@@ -77,7 +77,7 @@ MEASURED_TRAFFIC = {
 def measured_traffic(kernel, args):
     key = (kernel, args.width, args.height, 1 << args.agents_log2, args.storage,
            (args.resort, args.sort_cell_log2, args.trail_rows, args.agents_per_group,
-            args.streaming), args.steps, args.warmup)
+            args.streaming, args.sort_key), args.steps, args.warmup)
     kib = MEASURED_TRAFFIC.get(key)
     if kib is None or args.tiled or args.window or args.autotune_plain:
         return None
@@ -113,6 +113,9 @@ def parse():
                     help="batched/combined legs' re-sort interval before autotuning")
     ap.add_argument("--graph-chunk", type=int, default=100)
     ap.add_argument("--sort-cell-log2", type=int, default=3, help="re-sort bin edge 2^k cells")
+    ap.add_argument("--sort-key", type=lambda v: int(v, 0), default=0,
+                    help="plain leg's re-sort key refinement (slm_tuning.sort_key: heading "
+                         "classes log2 in bits 0-2, 0x10 = bin at the centre sensor)")
     ap.add_argument("--tiled", action="store_true",
                     help="plain leg: LDS-staged sensing from per-tile work lists")
     ap.add_argument("--window", action="store_true",
@@ -228,9 +231,11 @@ def roofline(name, bytes_per_launch, seconds):
 # What the counters show limits each kernel (rocprofv3 TA_BUSY, SQ_INSTS_VALU,
 # SQ_BUSY_CYCLES, FETCH_SIZE / WRITE_SIZE; profiles/agents_trail_c1_counters.md).
 LIMITER = {
-    "agents_kernel": ("texture-address (TA) processing of the divergent sensor gathers, then "
-                      "VALU; DRAM traffic is well below the algorithmic bytes because the taps "
-                      "hit L2, so the kernel is not HBM-bound"),
+    "agents_kernel": ("vector-L1 misses in flight against L2 latency on the divergent sensor "
+                      "gathers (the L1 waits on L2 data 62% of the time, the texture addresser "
+                      "waits on the L1 59%; profiles/agents_kernel_l1_counters.md), then VALU; "
+                      "DRAM traffic is well below the algorithmic bytes because the taps hit L2, "
+                      "so the kernel is not HBM-bound"),
     "trail_kernel": "HBM bandwidth",
 }
 
@@ -257,7 +262,7 @@ def plain_leg(slm, args):
                              sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled,
                              window_sensing=args.window)
         sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
-                       streaming=args.streaming)
+                       streaming=args.streaming, sort_key=args.sort_key)
         return sim
 
     sim = make()
@@ -302,7 +307,7 @@ def plain_leg(slm, args):
 
 
 TUNING_FIELDS = ("resort_interval", "sort_cell_log2", "agents_per_group", "trail_rows",
-                 "tiled_sensing", "instance_groups", "streaming")
+                 "tiled_sensing", "instance_groups", "streaming", "sort_key")
 
 
 def plain_gaussian_leg(slm, args, steps):
@@ -316,7 +321,7 @@ def plain_gaussian_leg(slm, args, steps):
                          resort_interval=args.resort, graph_chunk=args.graph_chunk,
                          sort_cell_log2=args.sort_cell_log2, gaussian=True)
     sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
-                   streaming=args.streaming)
+                   streaming=args.streaming, sort_key=args.sort_key)
     sim.step(args.warmup)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -436,7 +441,7 @@ def separate_leg(slm, specs, storage, steps, warmup, graph_chunk, tuning, serial
     chunk finishes before the next one's is enqueued (one simulation at a
     time, as a caller stepping one handle after another)."""
     fields = ("resort_interval", "sort_cell_log2", "agents_per_group", "trail_rows",
-              "tiled_sensing")
+              "tiled_sensing", "sort_key")
     log(f"separate: {len(specs)} simulations storage={storage} creating")
     sims = []
     try:

@@ -334,12 +334,24 @@ typedef struct slm_tuning {
                                 beyond the 256 MB Infinity Cache in f32
                                 (8192^2: 3 gives 75% of HBM peak against 72%
                                 with 0); costs on 16384^2 bf16 */
+  uint32_t sort_key;         /* re-sort key refinement: the agents of a bin
+                                are further split into 2^k heading classes
+                                (k = sort_key & SLM_SORT_HEADING_MASK, 0..5),
+                                and with SLM_SORT_AT_SENSOR the bin is the one
+                                holding the point the centre sensor reads
+                                instead of the agent's own cell.  Either
+                                gathers each wave's sensor taps onto fewer
+                                cache lines.  Not with tiled_sensing = 1
+                                (SLM_ERR_UNSUPPORTED). */
   /* autotune, for the chosen tuning: step_ms = device time per step from the
      first launch to the end of the last (eager launches, gaps included);
      agents/trail/sort_ms = per-step sums of those kernels' own times (sorts
      amortised over their interval); else 0 */
   double step_ms, agents_ms, trail_ms, sort_ms;
 } slm_tuning;
+
+#define SLM_SORT_HEADING_MASK 0x7u
+#define SLM_SORT_AT_SENSOR 0x10u
 
 int slm_get_tuning(const slm_sim* sim, slm_tuning* out);
 /* Applies a tuning.  Agent and trail state, the step count and the parameters

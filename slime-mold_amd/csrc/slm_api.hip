@@ -148,6 +148,7 @@ struct EngineConfig {
   uint32_t trail_rows = 0;  // 0: chosen per instance by plan_trail
   uint32_t instance_groups = 1;
   uint32_t streaming = 0;
+  uint32_t sort_key = 0;  // SLM_SORT_* refinement of the re-sort key
 };
 
 constexpr uint32_t kMaxInstanceGroups = 4096;
@@ -167,6 +168,11 @@ int check_tuning(const slm_tuning& t) {
   if (t.streaming > 3) return fail(SLM_ERR_INVALID_ARGUMENT, "streaming must be 0..3");
   if (t.tiled_sensing == 1 && t.instance_groups > 1)
     return fail(SLM_ERR_UNSUPPORTED, "instance groups do not combine with tiled sensing");
+  if ((t.sort_key & ~(SLM_SORT_HEADING_MASK | SLM_SORT_AT_SENSOR)) ||
+      (t.sort_key & SLM_SORT_HEADING_MASK) > 5u)
+    return fail(SLM_ERR_INVALID_ARGUMENT, "sort_key: heading bits 0..5, SLM_SORT_AT_SENSOR");
+  if (t.tiled_sensing == 1 && t.sort_key)
+    return fail(SLM_ERR_UNSUPPORTED, "tiled sensing needs the plain re-sort key");
   return SLM_OK;
 }
 
@@ -328,6 +334,7 @@ class Engine {
     p.pos64 = pos64() ? 1 : 0;
     p.xcd_remap = (cfg.flags & SLM_FLAG_NO_XCD_REMAP) ? 0 : 1;
     p.sort_cell_log2 = cfg.sort_cell_log2;
+    p.sort_key = cfg.sort_key;
     p.total_tiles = total_tiles_;
     p.tile_inst = tile_inst_d_;
     // the re-sort builds the tiled pass's work list only when that pass is on
@@ -699,6 +706,7 @@ class Engine {
                                                               : 0u;
     t.instance_groups = cfg.instance_groups;
     t.streaming = cfg.streaming;
+    t.sort_key = cfg.sort_key;
     return t;
   }
 
@@ -707,6 +715,8 @@ class Engine {
     if ((cfg.flags & SLM_FLAG_PERSISTENT) && t.instance_groups > 1)
       return fail(SLM_ERR_UNSUPPORTED, "persistent passes do not combine with instance groups");
     SLM_TRY(sync());
+    if (t.resort_interval > 0 && bins_for(t.sort_cell_log2, t.sort_key) > kMaxTotalBins)
+      return fail(SLM_ERR_INVALID_ARGUMENT, "too many re-sort bins: raise sort_cell_log2");
     if (!fits(t)) return fail(SLM_ERR_OUT_OF_MEMORY, "not enough device memory for this tuning");
     if (sorting() == (t.resort_interval > 0)) return apply_tuning(t);
     Snapshot snap;
@@ -847,12 +857,11 @@ class Engine {
     s = Snapshot{};
   }
 
-  uint64_t bins_for(uint32_t cell_log2) const {
+  uint64_t bins_for(uint32_t cell_log2, uint32_t sort_key) const {
     uint64_t bins = 0;
     for (const Instance& i : inst) {
       Instance c = i;
-      plan_bins(c, cell_log2);
-      bins += c.nbins;
+      bins += plan_bins(c, cell_log2) << (sort_key & SLM_SORT_HEADING_MASK);
     }
     return bins;
   }
@@ -863,7 +872,7 @@ class Engine {
     bool sort = t.resort_interval > 0;
     uint64_t need = 64ull << 20;
     if (sort) {
-      uint64_t bins = bins_for(t.sort_cell_log2);
+      uint64_t bins = bins_for(t.sort_cell_log2, t.sort_key);
       if (bins > kMaxTotalBins) return false;
       need += bins * 12ull;
     }
@@ -895,6 +904,7 @@ class Engine {
       cfg.trail_rows = t.trail_rows;
       cfg.instance_groups = t.instance_groups ? t.instance_groups : 1u;
       cfg.streaming = t.streaming;
+      cfg.sort_key = t.sort_key;
       cfg.flags &= ~(SLM_FLAG_TILED_SENSING | SLM_FLAG_WINDOW_SENSING);
       if (t.tiled_sensing == 1) cfg.flags |= SLM_FLAG_TILED_SENSING;
       if (t.tiled_sensing == 2) cfg.flags |= SLM_FLAG_WINDOW_SENSING;
@@ -1020,13 +1030,22 @@ class Engine {
         t.sort_cell_log2 = c;
         SLM_TRY(take(t));
       }
+      // re-sort key: heading classes, then binning at the centre sensor
+      for (uint32_t key : {3u, SLM_SORT_AT_SENSOR | 3u, SLM_SORT_AT_SENSOR | 5u}) {
+        if (key == best.sort_key || best.tiled_sensing == 1) continue;
+        slm_tuning t = best;
+        t.sort_key = key;
+        SLM_TRY(take(t));
+      }
     }
-    // sensing: global taps (0), tile work lists (1: needs re-sorting and one
-    // instance group), per-workgroup LDS windows (2)
+    // sensing: global taps (0), tile work lists (1: needs re-sorting with the
+    // plain key and one instance group), per-workgroup LDS windows (2)
     {
       const uint32_t cur = best.tiled_sensing;
       for (uint32_t m : {0u, 1u, 2u}) {
-        if (m == cur || (m == 1u && (!best.resort_interval || best.instance_groups > 1))) continue;
+        if (m == cur || (m == 1u && (!best.resort_interval || best.instance_groups > 1 ||
+                                     best.sort_key != 0u)))
+          continue;
         slm_tuning t = best;
         t.tiled_sensing = m;
         SLM_TRY(take(t));
@@ -1304,10 +1323,12 @@ class Engine {
       in.unit_off = static_cast<uint32_t>(units);
       units += in.units;
       uint32_t cl = cfg.sort_cell_log2;
-      plan_bins(in, cl);
-      in.bin_off = static_cast<uint32_t>(bins);
-      bins += in.nbins;
-      plan_tiles(in, cl, kTileLog2);
+      const uint64_t spatial = plan_bins(in, cl);
+      plan_tiles(in, cl, kTileLog2);  // spatial bins: tiled sensing keeps the plain key
+      const uint64_t keys = spatial << (cfg.sort_key & SLM_SORT_HEADING_MASK);
+      in.nbins = static_cast<uint32_t>(std::min<uint64_t>(keys, kMaxTotalBins + 1ull));
+      in.bin_off = static_cast<uint32_t>(std::min<uint64_t>(bins, kMaxTotalBins + 1ull));
+      bins += keys;
     }
     uint64_t tiles = 0, gtiles = 0;
     for (auto& in : inst) {

@@ -67,7 +67,7 @@ struct Instance {
   uint64_t trail_off;               // first element of this instance's trail
   uint32_t unit_off, units;         // trail-pass wave units [unit_off, unit_off + units)
   uint32_t tile_rows, ntx;          // strip tiles (ntx > 0) or 64-cell units (ntx == 0)
-  uint32_t bin_off;                 // re-sort bins [bin_off, bin_off + nbins)
+  uint32_t bin_off;                 // re-sort keys [bin_off, bin_off + nbins): bins x heading classes
   uint32_t nbins;
   uint32_t nbx, nby;                // bins across / down
   uint32_t tile_log2;               // sensing tiles of 2^k x 2^k cells (aligned bin squares)
@@ -96,6 +96,7 @@ struct Pool {
   int pos64;      // f64 positions
   int xcd_remap;  // group consecutive logical blocks on one XCD
   uint32_t sort_cell_log2;
+  uint32_t sort_key;           // heading classes (low bits) and binning at the centre sensor
   // tiled sensing (valid after a re-sort has built the work list)
   uint32_t total_tiles;
   const uint32_t* tile_inst;   // device: instance of each tile
@@ -212,8 +213,9 @@ inline uint32_t blocks_for(uint64_t n, uint32_t b) { return static_cast<uint32_t
 // rows > 0 fixes the tile height; 0 takes the tallest of 64..4 rows that still
 // gives target_waves units.
 void plan_trail(Instance& in, int storage, uint32_t target_waves, uint32_t rows);
-// Host-side layout of one instance's re-sort bins: fills nbins, nbx, nby.
-void plan_bins(Instance& in, uint32_t cell_log2);
+// Host-side layout of one instance's re-sort bins: fills nbins (clamped to
+// 2^31), nbx, nby, and returns the exact bin count.
+uint64_t plan_bins(Instance& in, uint32_t cell_log2);
 // Tiles of 2^tile_log2 cells (>= the bin edge) for tiled sensing: fills
 // tile_log2 and tiles.
 void plan_tiles(Instance& in, uint32_t cell_log2, uint32_t tile_log2);

@@ -28,12 +28,36 @@ __device__ inline uint32_t spread_bits(uint32_t v) {  // 16 bits -> even positio
   return v;
 }
 
-__device__ inline uint32_t bin_of(float x, float y, const Instance& in, uint32_t cell_log2) {
-  uint32_t bx = static_cast<uint32_t>(x) >> cell_log2;  // x in [0, width]
-  uint32_t by = static_cast<uint32_t>(y) >> cell_log2;
+// float -> u32 of a value that may lie below 0 or beyond 2^32 (clamped).
+__device__ inline uint32_t clamp_u32(float v) {
+  v = __builtin_fmaxf(v, 0.0f);
+  v = __builtin_fminf(v, 4294967040.0f);
+  return static_cast<uint32_t>(v);
+}
+
+// The re-sort key of an agent at (x, y) with heading th: the Morton code of
+// its 2^cell_log2 bin - with SLM_SORT_AT_SENSOR the bin holding the point its
+// centre sensor reads, (x, y) + sensor_dist * (cos th, sin th) - followed by
+// k = sort_key & SLM_SORT_HEADING_MASK bits of heading class.  The point only
+// has to be near the true one, so it comes from the hardware sin/cos
+// (v_sin/v_cos_f32, argument in revolutions).  The count and scatter kernels
+// evaluate this function on the same loaded values, so they agree on every
+// key.
+__device__ inline uint32_t key_of(float x, float y, float th, const Instance& in,
+                                  const DevParams& p, uint32_t cell_log2, uint32_t sort_key) {
+  const uint32_t hl = sort_key & SLM_SORT_HEADING_MASK;
+  const float rev = th * 0.159154943f;
+  if (sort_key & SLM_SORT_AT_SENSOR) {
+    x = __builtin_fmaf(p.sensor_dist, __builtin_amdgcn_cosf(rev), x);
+    y = __builtin_fmaf(p.sensor_dist, __builtin_amdgcn_sinf(rev), y);
+  }
+  uint32_t bx = clamp_u32(x) >> cell_log2;  // x in [0, width] without the sensor offset
+  uint32_t by = clamp_u32(y) >> cell_log2;
   bx = bx < in.nbx ? bx : in.nbx - 1u;
   by = by < in.nby ? by : in.nby - 1u;
-  return in.bin_off + (spread_bits(bx) | (spread_bits(by) << 1));
+  const uint32_t m = spread_bits(bx) | (spread_bits(by) << 1);
+  const uint32_t h = clamp_u32(rev * static_cast<float>(1u << hl)) & ((1u << hl) - 1u);
+  return in.bin_off + ((m << hl) | h);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
@@ -60,8 +84,9 @@ constexpr int kCountPer = 4;
 constexpr uint32_t kCountSlots = kCountPer * kAgentBlock;
 
 __global__ __launch_bounds__(kAgentBlock) void count_kernel(
-    AgentSoA a, const Instance* insts, const uint32_t* block_inst, uint32_t ninst, uint32_t total,
-    int pos64, uint32_t cell_log2, int xcd_remap, uint32_t* rank, uint32_t* count) {
+    AgentSoA a, const Instance* insts, const DevParams* params, const uint32_t* block_inst,
+    uint32_t ninst, uint32_t total, int pos64, uint32_t cell_log2, uint32_t sort_key,
+    int xcd_remap, uint32_t* rank, uint32_t* count) {
   __shared__ uint32_t s_min, s_max;
   __shared__ uint32_t s_cnt[kLocalBins];
   __shared__ uint32_t s_base[kLocalBins];
@@ -72,17 +97,20 @@ __global__ __launch_bounds__(kAgentBlock) void count_kernel(
       (xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x) * kCountSlots;
   uint32_t k[kCountPer];
   bool act[kCountPer];
-  float px[kCountPer], py[kCountPer];
+  float px[kCountPer], py[kCountPer], pth[kCountPer];
+  // the heading only enters refined keys: the plain key loads positions only
+  const bool heading = sort_key != 0u;
 #pragma unroll
   for (int q = 0; q < kCountPer; q++) {
     uint32_t j = base + q * kAgentBlock + threadIdx.x;
     // 64-bit test: near the 2^32 - 256 agent limit the last block's slots wrap
     act[q] = uint64_t(base) + q * kAgentBlock + threadIdx.x < total;
     k[q] = 0;
-    px[q] = py[q] = 0.0f;
+    px[q] = py[q] = pth[q] = 0.0f;
     if (act[q]) {
       px[q] = pos64 ? static_cast<float>(a.x64[j]) : a.x[j];
       py[q] = pos64 ? static_cast<float>(a.y64[j]) : a.y[j];
+      if (heading) pth[q] = a.th[j];
     }
   }
 #pragma unroll
@@ -94,7 +122,8 @@ __global__ __launch_bounds__(kAgentBlock) void count_kernel(
     for (uint32_t ii = block_inst[first / kAgentBlock];; ii++) {  // block-uniform walk
       const Instance in = insts[ii];
       uint32_t end = in.agent_off + in.agent_count;
-      if (act[q] && j >= in.agent_off && j < end) k[q] = bin_of(px[q], py[q], in, cell_log2);
+      if (act[q] && j >= in.agent_off && j < end)
+        k[q] = key_of(px[q], py[q], pth[q], in, params[ii], cell_log2, sort_key);
       if (last < end || ii + 1u >= ninst) break;
     }
   }
@@ -219,17 +248,18 @@ __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(uint32_t* out, uin
 // kernel's, so the same bin) instead of being stored by the count kernel and
 // read back: 8 bytes per agent less traffic, ~15% of the re-sort.
 __global__ __launch_bounds__(kAgentBlock) void scatter_kernel(
-    AgentSoA src, AgentSoA dst, const Instance* insts, const uint32_t* block_inst, uint32_t ninst,
-    uint32_t n, int pos64, uint32_t cell_log2, int xcd_remap, const uint32_t* rank,
-    const uint32_t* offset) {
+    AgentSoA src, AgentSoA dst, const Instance* insts, const DevParams* params,
+    const uint32_t* block_inst, uint32_t ninst, uint32_t n, int pos64, uint32_t cell_log2,
+    uint32_t sort_key, int xcd_remap, const uint32_t* rank, const uint32_t* offset) {
   const uint32_t blk = xcd_remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
   uint32_t first = blk * kAgentBlock;
   uint32_t last = min(first + kAgentBlock, n) - 1u;
   uint32_t j = first + threadIdx.x;
   bool act = j < n;
   double x64 = 0.0, y64 = 0.0;
-  float x = 0.0f, y = 0.0f;
+  float x = 0.0f, y = 0.0f, th = 0.0f;
   if (act) {
+    th = src.th[j];
     if (pos64) {
       x64 = src.x64[j];
       y64 = src.y64[j];
@@ -244,11 +274,12 @@ __global__ __launch_bounds__(kAgentBlock) void scatter_kernel(
   for (uint32_t ii = block_inst[blk];; ii++) {  // block-uniform walk
     const Instance in = insts[ii];
     uint32_t end = in.agent_off + in.agent_count;
-    if (act && j >= in.agent_off && j < end) k = bin_of(x, y, in, cell_log2);
+    if (act && j >= in.agent_off && j < end) k = key_of(x, y, th, in, params[ii], cell_log2, sort_key);
     if (last < end || ii + 1u >= ninst) break;
   }
   if (!act) return;
-  uint32_t d = offset[k] + rank[j];
+  const uint32_t d = offset[k] + rank[j];
+  if (d >= n) return;  // unreachable: count and scatter compute the same keys
   if (pos64) {
     dst.x64[d] = x64;
     dst.y64[d] = y64;
@@ -256,7 +287,7 @@ __global__ __launch_bounds__(kAgentBlock) void scatter_kernel(
     dst.x[d] = x;
     dst.y[d] = y;
   }
-  dst.th[d] = src.th[j];
+  dst.th[d] = th;
   dst.v[d] = src.v[j];
   dst.id[d] = src.id[j];
 }
@@ -335,12 +366,16 @@ void scan(const SortScratch& s, const uint32_t* in, uint32_t n, uint32_t* out, i
 
 }  // namespace
 
-void plan_bins(Instance& in, uint32_t cell_log2) {
+uint64_t plan_bins(Instance& in, uint32_t cell_log2) {
   uint32_t side = 1u << cell_log2;
   in.nbx = (in.width + side - 1) / side;
   in.nby = (in.height + side - 1) / side;
-  uint32_t p = next_pow2(in.nbx > in.nby ? in.nbx : in.nby);
-  in.nbins = p * p;
+  uint64_t p = next_pow2(in.nbx > in.nby ? in.nbx : in.nby);
+  // a wide grid with small bins can need 2^32 or more; the caller rejects
+  // counts beyond its limit (2^30), below which bin coordinates fit the
+  // 16-bit Morton code
+  in.nbins = static_cast<uint32_t>(std::min<uint64_t>(p * p, 0x80000000ull));
+  return p * p;
 }
 
 uint32_t plan_scan(SortScratch& s, uint32_t nbins) {
@@ -371,12 +406,12 @@ void launch_morton_sort(const Pool& pool, const AgentSoA& src, const AgentSoA& d
   if (pool.total_agents == 0) return;
   zero_u32_kernel<<<blocks_for(pool.total_bins, 256), 256, 0, stream>>>(s.count, pool.total_bins);
   count_kernel<<<blocks_for(pool.total_agents, kCountSlots), kAgentBlock, 0, stream>>>(
-      src, pool.inst, pool.agent_block_inst, pool.count, pool.total_agents, pool.pos64,
-      pool.sort_cell_log2, pool.xcd_remap, s.rank, s.count);
+      src, pool.inst, pool.params, pool.agent_block_inst, pool.count, pool.total_agents,
+      pool.pos64, pool.sort_cell_log2, pool.sort_key, pool.xcd_remap, s.rank, s.count);
   scan(s, s.count, pool.total_bins, s.offset, 0, stream);
   scatter_kernel<<<blocks_for(pool.total_agents, kAgentBlock), kAgentBlock, 0, stream>>>(
-      src, dst, pool.inst, pool.agent_block_inst, pool.count, pool.total_agents, pool.pos64,
-      pool.sort_cell_log2, pool.xcd_remap, s.rank, s.offset);
+      src, dst, pool.inst, pool.params, pool.agent_block_inst, pool.count, pool.total_agents,
+      pool.pos64, pool.sort_cell_log2, pool.sort_key, pool.xcd_remap, s.rank, s.offset);
   if (pool.items) {  // work list of the tiled agent pass
     uint32_t nt = pool.total_tiles;
     tile_count_kernel<<<blocks_for(nt, 256), 256, 0, stream>>>(

@@ -30,6 +30,8 @@ FLAG_STOCHASTIC_ROUNDING = 0x8
 FLAG_GAUSSIAN_DIFFUSE = 0x10
 FLAG_WINDOW_SENSING = 0x20
 FLAG_PERSISTENT = 0x40
+SORT_HEADING_MASK = 0x7
+SORT_AT_SENSOR = 0x10
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -149,6 +151,7 @@ class Tuning(ctypes.Structure):
         ("trials", ctypes.c_uint32),
         ("instance_groups", ctypes.c_uint32),
         ("streaming", ctypes.c_uint32),
+        ("sort_key", ctypes.c_uint32),
         ("step_ms", ctypes.c_double),
         ("agents_ms", ctypes.c_double),
         ("trail_ms", ctypes.c_double),

@@ -169,6 +169,27 @@ def test_instance_groups_equal_one_group(slm, orc, groups, chunk, resort, storag
             assert_bitwise(outs[1][k][1], t, f"instance {k} trail vs oracle")
 
 
+@pytest.mark.parametrize("sort_key,groups", [(3, 1), (0x13, 3), (0x15, 1), (0x10, 2)])
+def test_refined_sort_keys_track_the_oracle(slm, orc, sort_key, groups):
+    """Re-sorts keyed by heading class and by the centre sensor's bin (each
+    instance with its own sensor distance, up to 225 cells off small grids):
+    only the agents' slots change, so every instance still equals the oracle."""
+    specs = mixed_specs(slm, seed=6)
+    print(f"[sort key] {sort_key:#x} groups={groups}", flush=True)
+    with slm.Batch(specs, graph_chunk=5, resort_interval=3, sort_cell_log2=3) as b:
+        b.set_tuning(sort_key=sort_key, instance_groups=groups)
+        assert b.get_tuning().sort_key == sort_key
+        starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+        b.step(4)
+        b.step(13)
+        got = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+    for k, s in enumerate(specs):
+        a, t = starts[k]
+        orc.step(a, t, s.params, 17)
+        assert_bitwise(got[k][0], a, f"instance {k} agents")
+        assert_bitwise(got[k][1], t, f"instance {k} trail")
+
+
 def test_instance_groups_switched_mid_run(slm):
     specs = mixed_specs(slm, seed=4)
     with slm.Batch(specs, graph_chunk=4, resort_interval=3) as ref:

@@ -1,7 +1,7 @@
 """Launch tuning and the autotuner on the GPU.
 
 A tuning only changes how the step is laid out (agents per workgroup, trail
-tile rows, re-sort interval and bin size, tiled sensing), never a result bit:
+tile rows, re-sort interval, bin size and key, tiled sensing), never a result bit:
 every tuning, switched mid-run, must track the oracle exactly, and autotune
 must leave the state it found, so a run that autotunes halfway equals one that
 does not.
@@ -21,7 +21,11 @@ TUNINGS = [
     dict(trail_rows=64, streaming=3),
     dict(trail_rows=8, streaming=2),
     dict(resort_interval=3, sort_cell_log2=3),
-    dict(resort_interval=2, sort_cell_log2=5, tiled_sensing=1, agents_per_group=2048),
+    dict(resort_interval=3, sort_cell_log2=3, sort_key=3),
+    dict(resort_interval=2, sort_cell_log2=4, sort_key=0x15, tiled_sensing=2),
+    dict(resort_interval=5, sort_cell_log2=2, sort_key=0x10, tiled_sensing=0),
+    dict(resort_interval=2, sort_cell_log2=5, tiled_sensing=1, agents_per_group=2048,
+         sort_key=0),
     dict(resort_interval=0, tiled_sensing=1, trail_rows=8),
 ]
 
@@ -51,10 +55,15 @@ def test_set_tuning_rejects_bad_values(slm):
         for fields in [dict(agents_per_group=300), dict(agents_per_group=8192),
                        dict(agents_per_group=128), dict(trail_rows=3), dict(trail_rows=128),
                        dict(sort_cell_log2=13), dict(tiled_sensing=3),
-                       dict(instance_groups=4097), dict(streaming=4)]:
+                       dict(instance_groups=4097), dict(streaming=4), dict(sort_key=6),
+                       dict(sort_key=0x20), dict(sort_key=0x8)]:
             with pytest.raises(slm.SlimeMoldError) as e:
                 sim.set_tuning(**fields)
             assert e.value.code == -1, fields
+        # tiled sensing's tile work lists need the plain re-sort key
+        with pytest.raises(slm.SlimeMoldError) as e:
+            sim.set_tuning(resort_interval=4, tiled_sensing=1, sort_key=0x13)
+        assert e.value.code == -5
         assert sim.get_tuning().as_dict() == before
         sim.step(3)  # still usable
 
@@ -75,7 +84,7 @@ def test_autotune_restores_state(slm, orc, resort):
         assert_bitwise(sim.read_trail(), t, "trail after autotune")
         chosen = sim.get_tuning()
         for k in ("resort_interval", "sort_cell_log2", "agents_per_group", "trail_rows",
-                  "tiled_sensing"):
+                  "tiled_sensing", "sort_key"):
             assert getattr(chosen, k) == getattr(res, k), k
         sim.step(23)
         orc.step(a, t, s, 23)
@@ -167,3 +176,17 @@ def test_resort_schedule_is_absolute_across_call_sizes(slm):
         a.synchronize()
         assert_bitwise(b.read_agents(), a.read_agents(), "agents")
         assert_bitwise(b.read_trail(), a.read_trail(), "trail")
+
+
+def test_too_many_resort_bins_is_an_error(slm):
+    """A wide grid with small bins needs 2^40 bins here: creation (and a tuning
+    asking for it) fails with an error instead of wrapping the count."""
+    with pytest.raises(slm.SlimeMoldError) as e:
+        slm.Simulation(1 << 20, 64, 1000, seed=1, resort_interval=5, sort_cell_log2=0)
+    assert e.value.code == -1
+    with slm.Simulation(1 << 20, 64, 1000, seed=1, resort_interval=5, sort_cell_log2=10) as sim:
+        for fields in (dict(sort_cell_log2=0), dict(sort_cell_log2=4, sort_key=5)):
+            with pytest.raises(slm.SlimeMoldError) as e:
+                sim.set_tuning(**fields)
+            assert e.value.code == -1, fields
+        sim.step(6)  # still usable after the refusals
