@@ -121,8 +121,9 @@ typedef struct slm_config {
 /* With BF16 storage: the trail pass stores each cell rounded up with
  * probability equal to the discarded fraction of its f32 value, instead of to
  * nearest-even.  The decision adds the 16 discarded bits to 16 pseudo-random
- * bits, the top half of ((cell * 0x9e3779b1) ^ value bits) * 0x85ebca6b, so
- * runs stay deterministic and reproducible.  Unbiased: removes the drift of a
+ * bits: x = (cell * 0x9e3779b1) ^ value bits, then twice x ^= x >> 16 and
+ * x = (x mod 2^24) * K (K = 0xebca6b, then 0x9e3779), mod 2^32, and bits
+ * 16..31 of x; so runs stay deterministic and reproducible.  Unbiased: removes the drift of a
  * long round-to-nearest BF16 run's mass.  Other storage: SLM_ERR_UNSUPPORTED.
  * Values given to slm_write_trail are still rounded to nearest. */
 #define SLM_FLAG_STOCHASTIC_ROUNDING 0x8u

@@ -133,12 +133,22 @@ float orc_round_storage(float v, int storage) {
 }
 
 /* Stochastic rounding to bfloat16, deterministic: the 16 discarded bits of a
- * non-negative finite f32 are added to 16 pseudo-random bits, the top half of
- * ((cell * 0x9e3779b1) ^ bits) * 0x85ebca6b, and the sum truncated.  The
- * result rounds up with probability (discarded bits) / 2^16, so it is
- * unbiased. */
+ * non-negative finite f32 are added to 16 pseudo-random bits and the sum
+ * truncated.  The bits come from the cell and the value: x = (cell *
+ * 0x9e3779b1) ^ bits, then two rounds of x ^= x >> 16 and a multiply of x's
+ * low 24 bits by an odd 24-bit constant (0xebca6b, then 0x9e3779), all mod
+ * 2^32; the draw is bits 16..31 of the result.  The xor-shifts break the
+ * lattice a single multiplicative hash of the linear cell sequence leaves
+ * between neighbouring cells; the 24-bit products are full-rate
+ * v_mul_u32_u24 on the GPU.  The result rounds up with probability
+ * (discarded bits) / 2^16, so it is unbiased. */
 uint32_t orc_sr_bits(uint32_t cell, uint32_t bits) {
-  return (((cell * 0x9e3779b1u) ^ bits) * 0x85ebca6bu) >> 16;
+  uint32_t x = (cell * 0x9e3779b1u) ^ bits;
+  x ^= x >> 16;
+  x = (x & 0xffffffu) * 0xebca6bu;
+  x ^= x >> 16;
+  x = (x & 0xffffffu) * 0x9e3779u;
+  return x >> 16;
 }
 
 uint16_t orc_f32_to_bf16_sr(float v, uint32_t cell) {

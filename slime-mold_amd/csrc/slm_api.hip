@@ -1336,7 +1336,8 @@ class Engine {
       tiles += in.tiles;
       in.gtx = (in.width + kGaussTW - 1u) / kGaussTW;
       in.gtile_off = static_cast<uint32_t>(gtiles);
-      gtiles += static_cast<uint64_t>(in.gtx) * ((in.height + kGaussTH - 1u) / kGaussTH);
+      const uint32_t th = gauss_th(cfg.storage);
+      gtiles += static_cast<uint64_t>(in.gtx) * ((in.height + th - 1u) / th);
     }
     if (gtiles >= (1ull << 31)) return fail(SLM_ERR_INVALID_ARGUMENT, "too many trail tiles");
     total_gtiles_ = static_cast<uint32_t>(gtiles);
@@ -1550,7 +1551,8 @@ class Engine {
       G.units = z.unit_off + z.units - a.unit_off;
       G.trail_base = a.trail_off;
       G.gtile_base = a.gtile_off;
-      G.gtiles = z.gtile_off + z.gtx * ((z.height + kGaussTH - 1u) / kGaussTH) - a.gtile_off;
+      const uint32_t th = gauss_th(cfg.storage);
+      G.gtiles = z.gtile_off + z.gtx * ((z.height + th - 1u) / th) - a.gtile_off;
       std::vector<Instance> ins(inst.begin() + G.k0, inst.begin() + G.k1);
       bool first = true;
       for (Instance& in : ins) {

@@ -106,9 +106,13 @@ struct Codec<SLM_STORAGE_F16> {
 // bits are added to 16 pseudo-random bits and the sum truncated, so a value
 // rounds up with probability (discarded bits) / 2^16 - unbiased, where
 // round-to-nearest-even drifts a long run's mass by several percent.  The
-// random bits are the top half of ((cell * 0x9e3779b1) ^ bits) * 0x85ebca6b:
-// two multiplies (Knuth's and MurmurHash3's odd constants) that mix the cell
-// index and the value, so every cell and step draws differently.
+// random bits mix the cell index and the value, so every cell and step draws
+// differently: x = (cell * 0x9e3779b1) ^ bits, then twice x ^= x >> 16 and a
+// multiply of x's low 24 bits by an odd 24-bit constant (0xebca6b, 0x9e3779),
+// mod 2^32; the draw is bits 16..31.  The xor-shifts keep neighbouring cells'
+// draws independent (a single multiplicative hash of the linear cell sequence
+// correlates them); the 24-bit products are full-rate v_mul_u32_u24, where a
+// 32-bit multiply is quarter-rate.
 constexpr int kStorageBf16Sr = 3;
 
 // The first multiply of the draw, cell * kSrK1 (mod 2^32), is linear in the
@@ -116,10 +120,15 @@ constexpr int kStorageBf16Sr = 3;
 // step it with one add per cell instead of a quarter-rate 32-bit multiply
 // (the hashed cell `hc` below); the bits are the same.
 constexpr uint32_t kSrK1 = 0x9e3779b1u;
+constexpr uint32_t kSrK2 = 0xebca6bu, kSrK3 = 0x9e3779u;  // 24 bits each
 __device__ inline uint32_t bf16_sr_hashed(float v, uint32_t hc) {
   uint32_t b = __float_as_uint(v);  // v >= 0 and finite
-  uint32_t r = (hc ^ b) * 0x85ebca6bu;
-  return (b + (r >> 16)) >> 16;
+  uint32_t x = hc ^ b;
+  x ^= x >> 16;
+  x = __umul24(x, kSrK2);
+  x ^= x >> 16;
+  x = __umul24(x, kSrK3);
+  return (b + (x >> 16)) >> 16;
 }
 
 template <>
@@ -591,7 +600,7 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
 
 
 // ---- Gaussian diffuse (SLM_FLAG_GAUSSIAN_DIFFUSE) ------------------------------
-// One 256-thread workgroup per tile of kGaussTW x kGaussTH cells.  The tile's
+// One 256-thread workgroup per tile of kGaussTW x gauss_th(storage) cells.  The tile's
 // deposited and decayed cells plus an R-cell toroidal halo on every side are
 // staged in LDS (as 16-byte chunks where the tile lies inside a row of whole
 // chunks, else one cell per lane: see gauss_tile); thread c then runs the
@@ -607,8 +616,13 @@ struct GaussArgs {
   uint32_t count;  // instances
 };
 
-constexpr int kGaussRows = static_cast<int>(kGaussTH) + 2 * kGaussMaxR;
 constexpr int kGaussCols = static_cast<int>(kGaussTW) + 2 * kGaussMaxR;
+// tile rows of storage S (Codec<S>: the stochastic-rounding format is 16-bit)
+template <int S>
+constexpr int gauss_rows_of() {
+  return static_cast<int>(sizeof(typename Codec<S>::elem) == 4 ? gauss_th(SLM_STORAGE_F32)
+                                                                : gauss_th(SLM_STORAGE_BF16));
+}
 
 // v mod n for |v| < 2^25 and 1 <= n <= 2^24 (grid sides are at most 2^24).
 __device__ inline uint32_t wrap_index(int32_t v, uint32_t n) {
@@ -622,25 +636,44 @@ __device__ inline uint32_t wrap_near(int32_t v, uint32_t n) {
   return static_cast<uint32_t>(v < 0 ? v + m : (v >= m ? v - m : v));
 }
 
+// deposit_decay for the Gaussian pass's staging.  kSat: deposit >= 1 (every
+// preset), so a marked cell's clamp(|T| + deposit, 0, 1) is exactly 1 and the
+// cell becomes sat = max(1 - decay, 0), computed once with the same two
+// operations; an unmarked cell is max(|T| - decay, 0) as always.  Same bits,
+// one select instead of an add and a select per staged cell.
+template <int S, bool kSat>
+__device__ inline float gauss_stage(uint32_t raw, const DevParams& p, float sat) {
+  if constexpr (kSat) {
+    const float u = Codec<S>::value(raw) - p.decay_k;
+    const float d = u > 0.0f ? u : 0.0f;
+    return Codec<S>::marked(raw) ? sat : d;
+  } else {
+    return deposit_decay<S>(raw, p);
+  }
+}
+
 // One tile at radius R (a compile-time constant, so every loop unrolls and the
 // weights sit in registers).  A tile whose 256 columns lie inside the row, in
 // a row of whole 16-byte chunks, moves 16 bytes per lane: the rows are loaded
 // as chunks (plus single halo cells), and the results are staged in LDS and
 // stored as chunks.  Other tiles load and store one cell per lane.  Both give
 // the same bits.
-template <int S, int R>
+template <int S, int R, bool kSat>
 __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
                                   const typename Codec<S>::elem* src, typename Codec<S>::elem* dst,
                                   uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
                                   const DevParams& p, const float* wts) {
   using elem = typename Codec<S>::elem;
-  constexpr int rows = static_cast<int>(kGaussTH) + 2 * R;
+  constexpr int kGaussTH = gauss_rows_of<S>();
+  constexpr int rows = kGaussTH + 2 * R;
   constexpr int cols = static_cast<int>(kGaussTW) + 2 * R;
   constexpr int V = static_cast<int>(16 / sizeof(elem));  // cells per 16-byte chunk
   constexpr int CPR = static_cast<int>(kGaussTW) / V;     // chunks per tile row
   float w[R + 1];
 #pragma unroll
   for (int i = 0; i <= R; i++) w[i] = wts[i];
+  const float sat_u = 1.0f - p.decay_k;
+  const float sat = sat_u > 0.0f ? sat_u : 0.0f;
   const int c = static_cast<int>(threadIdx.x);
   const int32_t ix0 = static_cast<int32_t>(x0), iy0 = static_cast<int32_t>(y0);
   const bool vec = W % V == 0 && x0 + kGaussTW <= W;  // block-uniform
@@ -679,7 +712,8 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
       if (q < kChunks) {
         const int r = q / CPR, j = q % CPR;
 #pragma unroll
-        for (int e = 0; e < V; e++) s_t[r][R + j * V + e] = deposit_decay<S>(Codec<S>::lane(raw[k], e), p);
+        for (int e = 0; e < V; e++)
+          s_t[r][R + j * V + e] = gauss_stage<S, kSat>(Codec<S>::lane(raw[k], e), p, sat);
       }
     }
 #pragma unroll
@@ -687,7 +721,7 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
       const int q = c + k * static_cast<int>(kGaussTW);
       if (q < kHalo) {
         const int r = q / (2 * R), jj = q % (2 * R);
-        s_t[r][jj < R ? jj : static_cast<int>(kGaussTW) + jj] = deposit_decay<S>(hraw[k], p);
+        s_t[r][jj < R ? jj : static_cast<int>(kGaussTW) + jj] = gauss_stage<S, kSat>(hraw[k], p, sat);
       }
     }
   } else {
@@ -706,8 +740,8 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
     }
 #pragma unroll
     for (int r = 0; r < rows; r++) {
-      s_t[r][c] = deposit_decay<S>(raw0[r], p);
-      if (second) s_t[r][c + kGaussTW] = deposit_decay<S>(raw1[r], p);
+      s_t[r][c] = gauss_stage<S, kSat>(raw0[r], p, sat);
+      if (second) s_t[r][c + kGaussTW] = gauss_stage<S, kSat>(raw1[r], p, sat);
     }
   }
   __syncthreads();
@@ -729,13 +763,12 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
     uint32_t o[kGaussTH];
 #pragma unroll
     for (int r = 0; r < static_cast<int>(kGaussTH); r++) {
-      const uint32_t y = y0 + static_cast<uint32_t>(r);
       float g = w[R] * h[r];
 #pragma unroll
       for (int j = 1; j <= 2 * R; j++) g = __builtin_fmaf(w[j < R ? R - j : j - R], h[r + j], g);
       float l = s_t[r + R][c + R] * p.one_minus_rate;
       float rr = g * p.rate;
-      o[r] = y < H ? encode_out<S>(l + rr, hc) : 0u;
+      o[r] = encode_out<S>(l + rr, hc);  // rows past the map are never stored
       hc += hstep;
     }
     __syncthreads();  // every read of s_t is done: it becomes the output stage
@@ -776,7 +809,7 @@ template <int S>
 __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
                                                          const Instance* __restrict__ insts,
                                                          const DevParams* __restrict__ params) {
-  __shared__ __align__(16) float s_t[kGaussRows][kGaussCols];
+  __shared__ __align__(16) float s_t[gauss_rows_of<S>() + 2 * kGaussMaxR][kGaussCols];
   using elem = typename Codec<S>::elem;
   const uint32_t t = blockIdx.x;
   uint32_t lo = 0, hi = a.count - 1u;  // last instance whose tiles start at or before t
@@ -790,22 +823,30 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
   const Instance in = insts[lo];
   const DevParams p = params[lo];
   const uint32_t lt = t - in.gtile_off;
-  const uint32_t x0 = (lt % in.gtx) * kGaussTW, y0 = (lt / in.gtx) * kGaussTH;
+  const uint32_t x0 = (lt % in.gtx) * kGaussTW, y0 = (lt / in.gtx) * gauss_rows_of<S>();
   const elem* src = static_cast<const elem*>(a.in) + in.trail_off;
   elem* dst = static_cast<elem*>(a.out) + in.trail_off;
   const float* wts = params[lo].gauss_w;
   // 1..kGaussMaxR (the host checks it); any other value writes nothing
-  switch (p.gauss_r) {
-    case 1: gauss_tile<S, 1>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    case 2: gauss_tile<S, 2>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    case 3: gauss_tile<S, 3>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    case 4: gauss_tile<S, 4>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    case 5: gauss_tile<S, 5>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    case 6: gauss_tile<S, 6>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    case 7: gauss_tile<S, 7>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    case 8: gauss_tile<S, 8>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-    default: break;
-  }
+  auto run = [&](auto sat) {
+    constexpr bool kSat = decltype(sat)::value;
+    switch (p.gauss_r) {
+      case 1: gauss_tile<S, 1, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 2: gauss_tile<S, 2, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 3: gauss_tile<S, 3, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 4: gauss_tile<S, 4, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 5: gauss_tile<S, 5, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 6: gauss_tile<S, 6, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 7: gauss_tile<S, 7, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 8: gauss_tile<S, 8, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      default: break;
+    }
+  };
+  // deposit >= 1 (every preset) takes the one-select staging (same bits)
+  if (p.deposit >= 1.0f)
+    run(std::true_type{});
+  else
+    run(std::false_type{});
 }
 
 template <int S>

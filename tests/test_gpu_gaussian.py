@@ -55,6 +55,28 @@ def test_gaussian_single_matches_oracle(slm, orc, case):
     assert_bitwise(gt, t, f"case {case} trail")
 
 
+@pytest.mark.parametrize("deposit,storage,w,h", [
+    (0.3, "bf16sr", 512, 64), (0.3, "f32", 300, 50), (2.5, "bf16sr", 512, 64),
+    (1.0, "f16", 256, 40), (0.75, "bf16", 260, 33)])
+def test_gaussian_deposit_amounts(slm, orc, deposit, storage, w, h):
+    """Deposits below 1 take the general deposit + clamp staging; 1 and above
+    the one-select form (a marked cell becomes exactly 1 before the decay);
+    whole-tile and ragged maps, in every storage."""
+    s = dataclasses.replace(preset(slm, "Default"), pheromone_deposition_amount=deposit,
+                            blur_radius=2.0, blur_sigma=1.0, pheromone_diffusion_rate=0.85)
+    print(f"[gauss deposit] {deposit} {storage} {w}x{h}", flush=True)
+    rng = np.random.default_rng(int(deposit * 10) + w)
+    with slm.Simulation(w, h, w * h // 2, seed=9, params=s, graph_chunk=3, gaussian=True,
+                        **storage_kw(storage)) as sim:
+        sim.write_trail(rng.uniform(0.0, 1.0, (h, w)).astype(np.float32))
+        a, t = sim.read_agents(), sim.read_trail()
+        sim.step(7)
+        ga, gt = sim.read_agents(), sim.read_trail()
+    orc.step(a, t, s, 7, storage=storage, gaussian=True)
+    assert_bitwise(ga, a, "agents")
+    assert_bitwise(gt, t, "trail")
+
+
 @pytest.mark.parametrize("groups", [1, 3])
 def test_gaussian_batch_radius_per_instance(slm, orc, groups):
     rng = np.random.default_rng(40 + groups)

@@ -162,3 +162,53 @@ def test_stochastic_bf16_rounding(orc):
                for c in range(1000))
     base = np.float32(1.0) + np.float32(0.3 * 2 ** -7)
     assert L.orc_f32_to_bf16(float(base)) == int(base.view(np.uint32)) >> 16  # nearest: down
+
+
+def test_stochastic_rounding_neighbours_draw_independently(orc):
+    """The random bits of horizontally and vertically adjacent cells (rows of
+    4096 and 16384 cells) are uncorrelated and jointly uniform, for a fixed
+    value and for the value a smooth field would hold in both cells; a
+    plausible slip (a dropped cell term, a multiply by an even constant, the
+    wrong half of the product) makes neighbours draw alike and fails here."""
+    L = orc.lib()
+    rng = np.random.default_rng(3)
+    cells = rng.integers(0, (1 << 28) - (1 << 15), 50000)
+    for bits in (0x3F800000, 0x3E4CCCCD, 0x3F3504F3):
+        r0 = np.array([L.orc_sr_bits(int(c), bits) for c in cells], np.float64)
+        for step in (1, 4096, 16384):
+            r1 = np.array([L.orc_sr_bits(int(c) + step, bits) for c in cells], np.float64)
+            rho = np.corrcoef(r0, r1)[0, 1]
+            assert abs(rho) < 0.02, (hex(bits), step, rho)
+            h = np.histogram2d(r0 // 4096, r1 // 4096, bins=16, range=[[0, 16], [0, 16]])[0]
+            e = len(cells) / 256.0
+            chi2 = float(((h - e) ** 2 / e).sum())
+            assert chi2 < 255 + 5 * np.sqrt(2 * 255), (hex(bits), step, chi2)
+
+
+def test_stochastic_rounding_is_unbiased_over_random_values(orc):
+    """Random values in [0.25, 1), one per cell: the stochastically rounded
+    values average to the values within four standard errors of the mean.
+    On values whose discarded fraction lies in [0, 1/2) round-to-nearest-even
+    always rounds down (mean error about -1/4 ulp, many standard errors from
+    zero); stochastic rounding of the same values stays unbiased."""
+    L = orc.lib()
+    rng = np.random.default_rng(11)
+    n = 40000
+
+    def sr_error(vals):
+        got = np.array([L.orc_f32_to_bf16_sr(float(x), k) for k, x in enumerate(vals)], np.uint32)
+        return (got << 16).view(np.float32).astype(np.float64) - vals.astype(np.float64)
+
+    def se(e):
+        return float(e.std()) / np.sqrt(len(e))
+
+    v = rng.uniform(0.25, 1.0, n).astype(np.float32)
+    err = sr_error(v)
+    assert abs(err.mean()) < 4 * se(err), (err.mean(), se(err))
+    low = ((v.view(np.uint32) & 0xFFFF0000) |
+           rng.integers(0, 0x8000, n).astype(np.uint32)).view(np.float32)
+    rne = np.array([L.orc_f32_to_bf16(float(x)) for x in low], np.uint32)
+    rne_err = (rne << 16).view(np.float32).astype(np.float64) - low.astype(np.float64)
+    sr_low = sr_error(low)
+    assert abs(sr_low.mean()) < 4 * se(sr_low), (sr_low.mean(), se(sr_low))
+    assert rne_err.mean() < -50 * se(sr_low), (rne_err.mean(), se(sr_low))
