@@ -17,9 +17,10 @@ f32 simulations with the batch's tuning, stepped one simulation at a time
 ("batched_vs_separate_concurrent").
 
 The plain leg runs a fixed launch tuning (flags; by default 16-row trail
-tiles, a re-sort every 15 steps into 8x8-cell bins and non-temporal trail-pass
-loads and stores; the library's autotuner found the bins, and 2000-step sweeps
-chose the interval and the streaming mode).
+tiles, a re-sort every 15 steps into 8x8-cell bins, 1024 agents per agent-pass
+workgroup and non-temporal trail-pass loads and stores; the library's
+autotuner found the bins, and sweeps over 20, 300 and 2000 steps chose the
+interval, the workgroup size and the streaming mode).
 Before its warmup the batched (or combined) leg lets the library's autotuner
 choose its tuning (re-sort interval and bin size, agents per workgroup, trail
 tile rows, tiled sensing, instance groups, streaming) on its own
@@ -67,10 +68,10 @@ WRITE_SIZE_SCALE = 1.0
 # 2000 after 100).
 MEASURED_TRAFFIC = {
     # (kernel, W, H, N, storage, tuning, steps, warmup): (FETCH KiB, WRITE KiB)
-    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3, 0), 20, 5):
-        (183378, 272155),
-    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 256, 3, 0), 2000, 100):
-        (169008, 218199),
+    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 1024, 3, 0), 20, 5):
+        (181418, 276013),
+    ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 1024, 3, 0), 2000, 100):
+        (169218, 219345),
 }
 
 
@@ -129,7 +130,8 @@ def parse():
                          "launch each per step over every instance; no instance groups)")
     ap.add_argument("--trail-rows", type=int, default=16,
                     help="plain leg: rows per trail-pass tile (0: chosen from the grid)")
-    ap.add_argument("--agents-per-group", type=int, default=256)
+    ap.add_argument("--agents-per-group", type=int, default=1024,
+                    help="plain leg: agents per workgroup of the agent pass (256..4096)")
     ap.add_argument("--streaming", type=int, default=3, choices=[0, 1, 2, 3],
                     help="plain leg: non-temporal trail-pass stores (1), loads (2) or both (3)")
     ap.add_argument("--autotune", type=int, default=80,
