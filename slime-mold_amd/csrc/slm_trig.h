@@ -76,25 +76,21 @@ SLM_TRIG_FN double slm_f64_from_bits(uint64_t u) {
 /*
  * Bits of 2/pi after the binary point, as a 288-bit string preceded by one zero
  * word: word 0 stands for the (zero) bits -31..0, words 1..8 hold bits 1..256.
- * Returned by index so that the table needs no device-side global.
  */
-SLM_TRIG_FN uint32_t slm_two_over_pi_word(int i) {
-  return i == 1 ? 0xa2f9836eu
-       : i == 2 ? 0x4e441529u
-       : i == 3 ? 0xfc2757d1u
-       : i == 4 ? 0xf534ddc0u
-       : i == 5 ? 0xdb629599u
-       : i == 6 ? 0x3c439041u
-       : i == 7 ? 0xfe5163abu
-       : i == 8 ? 0xdebbc561u
-       : 0u;
-}
+#define SLM_2OPI_W1 0xa2f9836eu
+#define SLM_2OPI_W2 0x4e441529u
+#define SLM_2OPI_W3 0xfc2757d1u
+#define SLM_2OPI_W4 0xf534ddc0u
+#define SLM_2OPI_W5 0xdb629599u
+#define SLM_2OPI_W6 0x3c439041u
+#define SLM_2OPI_W7 0xfe5163abu
+#define SLM_2OPI_W8 0xdebbc561u
 
-/* 32 bits of the extended 2/pi string starting at bit position p (0-based). */
-SLM_TRIG_FN uint32_t slm_two_over_pi_bits32(int p) {
-  int w = p >> 5, s = p & 31;
-  uint64_t pair = ((uint64_t)slm_two_over_pi_word(w) << 32) | slm_two_over_pi_word(w + 1);
-  return (uint32_t)(pair >> (32 - s));
+/* 32 bits of the string starting s bits into word a, b the next word
+ * (0 <= s < 32): the top half of (a:b) << s. */
+SLM_TRIG_FN uint32_t slm_funnel32(uint32_t a, uint32_t b, int s) {
+  uint64_t pair = ((uint64_t)a << 32) | b;
+  return (uint32_t)((pair << s) >> 32);
 }
 
 /*
@@ -106,11 +102,31 @@ SLM_TRIG_FN float slm_reduce_payne_hanek(float ax, int* q_out) {
   int e = (int)(ux >> 23) - 150;            /* ax = m * 2^e */
   uint64_t m = (ux & 0x7fffffu) | 0x800000u;
   /* Bits k <= e-2 of 2/pi contribute multiples of 4 to ax*2/pi: start the
-   * 96-bit window at bit k0 = e-1, i.e. extended position p = k0 + 31. */
+   * 96-bit window at bit k0 = e-1, i.e. extended position p = k0 + 31
+   * (19 <= p <= 134 for finite ax >= 4096, so words w .. w+3 with w <= 4). */
   int p = e + 30;
-  uint64_t hi = slm_two_over_pi_bits32(p);
-  uint64_t mid = slm_two_over_pi_bits32(p + 32);
-  uint64_t lo = slm_two_over_pi_bits32(p + 64);
+  int w = p >> 5, s = p & 31;
+  /* words w .. w+3, picked by selects on w (no table in memory, no branches) */
+  uint32_t a0 = 0u, a1 = SLM_2OPI_W1, a2 = SLM_2OPI_W2, a3 = SLM_2OPI_W3;
+  a0 = w == 1 ? SLM_2OPI_W1 : a0;
+  a1 = w == 1 ? SLM_2OPI_W2 : a1;
+  a2 = w == 1 ? SLM_2OPI_W3 : a2;
+  a3 = w == 1 ? SLM_2OPI_W4 : a3;
+  a0 = w == 2 ? SLM_2OPI_W2 : a0;
+  a1 = w == 2 ? SLM_2OPI_W3 : a1;
+  a2 = w == 2 ? SLM_2OPI_W4 : a2;
+  a3 = w == 2 ? SLM_2OPI_W5 : a3;
+  a0 = w == 3 ? SLM_2OPI_W3 : a0;
+  a1 = w == 3 ? SLM_2OPI_W4 : a1;
+  a2 = w == 3 ? SLM_2OPI_W5 : a2;
+  a3 = w == 3 ? SLM_2OPI_W6 : a3;
+  a0 = w == 4 ? SLM_2OPI_W4 : a0;
+  a1 = w == 4 ? SLM_2OPI_W5 : a1;
+  a2 = w == 4 ? SLM_2OPI_W6 : a2;
+  a3 = w == 4 ? SLM_2OPI_W7 : a3;
+  uint64_t hi = slm_funnel32(a0, a1, s);
+  uint64_t mid = slm_funnel32(a1, a2, s);
+  uint64_t lo = slm_funnel32(a2, a3, s);
   /* (m * window) mod 2^96 = (ax*2/pi mod 4) in fixed point with 94 fraction bits */
   uint64_t acc = m * lo;
   acc = (acc >> 32) + m * mid;
