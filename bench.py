@@ -80,7 +80,7 @@ def measured_traffic(kernel, args):
            (args.resort, args.sort_cell_log2, args.trail_rows, args.agents_per_group,
             args.streaming, args.sort_key), args.steps, args.warmup)
     kib = MEASURED_TRAFFIC.get(key)
-    if kib is None or args.tiled or args.window or args.autotune_plain:
+    if kib is None or args.tiled or args.window or args.autotune_plain or args.f64_positions:
         return None
     return int((kib[0] * FETCH_SIZE_SCALE + kib[1] * WRITE_SIZE_SCALE) * 1024)
 
@@ -108,6 +108,8 @@ def parse():
     ap.add_argument("--agents-log2", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
+    ap.add_argument("--f64-positions", action="store_true",
+                    help="plain leg: f64 agent positions (SLM_FLAG_F64_POSITIONS)")
     ap.add_argument("--resort", type=int, default=15,
                     help="plain leg's Morton re-sort interval (0: off)")
     ap.add_argument("--batch-resort", type=int, default=20,
@@ -155,7 +157,8 @@ def parse():
                          "batched leg, 0: skip)")
     ap.add_argument("--batch-storage", default="bf16", choices=["f32", "bf16", "f16"])
     ap.add_argument("--rounding", default="stochastic", choices=["nearest", "stochastic"],
-                    help="bf16 rounding of the batched, combined and bf16 diffuse legs")
+                    help="bf16 rounding of the batched, combined and bf16 diffuse legs, and "
+                         "of the plain legs when --storage bf16")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -251,6 +254,11 @@ def graph_split(segments, chunk):
     return g, sum(segments) - g
 
 
+def plain_sr(args):
+    """Stochastic rounding of the plain leg's stores: bf16 storage with --rounding stochastic."""
+    return args.storage == "bf16" and args.rounding == "stochastic"
+
+
 def plain_leg(slm, args):
     presets = [slm.workloads.preset(n) for n in slm.workloads.PLAIN_PRESETS]
     W, H, N = args.width, args.height, 1 << args.agents_log2
@@ -262,7 +270,8 @@ def plain_leg(slm, args):
         sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
                              resort_interval=args.resort, graph_chunk=args.graph_chunk,
                              sort_cell_log2=args.sort_cell_log2, tiled_sensing=args.tiled,
-                             window_sensing=args.window)
+                             window_sensing=args.window, f64_positions=args.f64_positions,
+                             stochastic_rounding=plain_sr(args))
         sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
                        streaming=args.streaming, sort_key=args.sort_key)
         return sim
@@ -270,7 +279,9 @@ def plain_leg(slm, args):
     sim = make()
     log(f"plain {W}x{H} agents={N} storage={args.storage} resort={args.resort} "
         f"graph_chunk={args.graph_chunk} warmup={args.warmup} steps={K}")
-    state0 = (sim.read_agents(), sim.read_trail()) if args.cpu_seconds > 0 else None
+    state0 = ((sim.read_agents(), sim.read_trail())
+              if args.cpu_seconds > 0 and args.storage == "f32" and not args.f64_positions
+              else None)
     tuning = autotune(sim, args.autotune if args.autotune_plain else 0, "plain")
     sim.step(args.warmup)
     sim.synchronize()
@@ -297,7 +308,7 @@ def plain_leg(slm, args):
     m_elapsed = time.perf_counter() - t0
     rep.close()
     log(f"plain: the same {K} steps again with per-kernel events in {m_elapsed:.3f} s")
-    a_bytes, t_bytes = agent_bytes(sb) * N, trail_bytes(sb) * W * H
+    a_bytes, t_bytes = agent_bytes(sb, args.f64_positions) * N, trail_bytes(sb) * W * H
     return {
         "value": N * K / elapsed, "elapsed": elapsed, "steps": K, "segments": seg,
         "measure_value": N * K / m_elapsed,
@@ -321,7 +332,8 @@ def plain_gaussian_leg(slm, args, steps):
     seg = [steps // 3, steps // 3, steps - 2 * (steps // 3)]
     sim = slm.Simulation(W, H, N, seed=args.seed, params=presets[0], storage=args.storage,
                          resort_interval=args.resort, graph_chunk=args.graph_chunk,
-                         sort_cell_log2=args.sort_cell_log2, gaussian=True)
+                         sort_cell_log2=args.sort_cell_log2, gaussian=True,
+                         f64_positions=args.f64_positions, stochastic_rounding=plain_sr(args))
     sim.set_tuning(trail_rows=args.trail_rows, agents_per_group=args.agents_per_group,
                    streaming=args.streaming, sort_key=args.sort_key)
     sim.step(args.warmup)
@@ -557,6 +569,8 @@ def main():
                          f"captured {args.graph_chunk}-step hipGraphs, {eager_steps} launched "
                          f"eagerly (a segment's steps beyond its last whole chunk)"),
             "width": W, "height": H, "agents": N, "storage": args.storage, "seed": args.seed,
+            "f64_positions": args.f64_positions,
+            "rounding": "stochastic" if plain_sr(args) else "nearest",
             "presets": slm.workloads.PLAIN_PRESETS, "graph_chunk": args.graph_chunk,
             "graph_steps": graph_steps, "eager_steps": eager_steps,
             "tuning": plain["tuning"],
@@ -628,7 +642,10 @@ def main():
                 out["separate"][mode] = sres
                 out[f"batched_vs_separate_{mode}"] = round(
                     bres["agent_updates_per_s"] / sres["agent_updates_per_s"], 3)
-    if args.cpu_seconds > 0:
+    if args.cpu_seconds > 0 and (args.storage != "f32" or args.f64_positions):
+        out["cpu_baseline_note"] = ("the CPU sample runs the f32 plain path only; this line's "
+                                    "storage or positions differ, so none is reported")
+    elif args.cpu_seconds > 0:
         import oracle
 
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
