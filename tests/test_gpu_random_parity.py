@@ -4,9 +4,9 @@ Each case draws a grid shape (1 cell to 1100 per side, ragged widths
 included), an agent count (0 to 2 per cell), a preset with every swept
 parameter perturbed (diffusion rates outside [0, 1], zero or large sensor
 distances, jitter on or off), a storage format, optionally a random starting
-trail, and a launch tuning (re-sort interval and bin size, tiled sensing,
-agents per workgroup, trail tile rows, streaming loads and stores, graph
-chunk length;
+trail, and a launch tuning (re-sort interval, bin size and key, tiled or windowed
+sensing, agents per workgroup, trail tile rows, streaming loads and stores,
+graph chunk length;
 for batches the instance groups).  A tuning never changes a result bit, so
 every case must equal the oracle exactly.  The case is printed before it runs.
 """
@@ -43,7 +43,10 @@ def random_settings(slm, rng):
     )
 
 
-def random_tuning(rng, batch):
+def random_tuning(rng, batch, extra=None):
+    """A random launch tuning.  `extra` (its own generator, so the draws of
+    `rng` and hence the shapes and parameters of every case stay put) adds
+    the newer options: refined re-sort keys and windowed sensing."""
     resort = int(rng.choice([0, 1, 3, 7]))
     t = dict(resort_interval=resort,
              sort_cell_log2=int(rng.integers(2, 7)),
@@ -53,7 +56,16 @@ def random_tuning(rng, batch):
              tiled_sensing=int(resort > 0 and rng.random() < 0.3))
     if batch and not t["tiled_sensing"]:
         t["instance_groups"] = int(rng.choice([1, 2, 3, 8]))
+    if extra is not None:
+        if not t["tiled_sensing"] and extra.random() < 0.2:
+            t["tiled_sensing"] = 2  # windowed sensing
+        if resort > 0 and t["tiled_sensing"] != 1 and extra.random() < 0.5:
+            t["sort_key"] = int(extra.choice([3, 5, 0x10, 0x13, 0x15]))
     return t
+
+
+def extra_rng(kind, case):
+    return np.random.default_rng([kind, case, 7])
 
 
 def storage_kw(storage):
@@ -68,7 +80,7 @@ def test_random_single_simulation(slm, orc, case):
     n = int(rng.integers(0, min(2 * w * h, 30000) + 1))
     s = random_settings(slm, rng)
     storage = STORAGES[rng.integers(len(STORAGES))]
-    tun = random_tuning(rng, batch=False)
+    tun = random_tuning(rng, batch=False, extra=extra_rng(1, case))
     chunk = int(rng.choice([0, 1, 5, 16]))
     steps = int(rng.integers(1, 13))
     seed = int(rng.integers(0, 2**31))
@@ -101,7 +113,7 @@ def test_random_batch(slm, orc, case):
         specs.append(slm.InstanceSpec(w, h, n, seed=int(rng.integers(0, 2**31)),
                                       params=random_settings(slm, rng)))
     storage = STORAGES[rng.integers(len(STORAGES))]
-    tun = random_tuning(rng, batch=True)
+    tun = random_tuning(rng, batch=True, extra=extra_rng(2, case))
     chunk = int(rng.choice([0, 3, 10]))
     steps = int(rng.integers(1, 11))
     print(f"[random batch {case}] {[(sp.width, sp.height, sp.agent_count) for sp in specs]} "
@@ -135,7 +147,7 @@ def test_random_gaussian_single(slm, orc, case):
     n = int(rng.integers(0, min(2 * w * h, 30000) + 1))
     s = gaussian_settings(slm, rng)
     storage = STORAGES[rng.integers(len(STORAGES))]
-    tun = random_tuning(rng, batch=False)
+    tun = random_tuning(rng, batch=False, extra=extra_rng(3, case))
     chunk = int(rng.choice([0, 1, 5, 16]))
     steps = int(rng.integers(1, 9))
     seed = int(rng.integers(0, 2**31))
@@ -166,7 +178,7 @@ def test_random_gaussian_batch(slm, orc, case):
         specs.append(slm.InstanceSpec(w, h, n, seed=int(rng.integers(0, 2**31)),
                                       params=gaussian_settings(slm, rng)))
     storage = STORAGES[rng.integers(len(STORAGES))]
-    tun = random_tuning(rng, batch=True)
+    tun = random_tuning(rng, batch=True, extra=extra_rng(4, case))
     chunk = int(rng.choice([0, 3, 10]))
     steps = int(rng.integers(1, 9))
     print(f"[random gaussian batch {case}] {[(sp.width, sp.height, sp.agent_count) for sp in specs]} "
