@@ -348,6 +348,7 @@ class Engine {
     p.stochastic_rounding = (cfg.flags & SLM_FLAG_STOCHASTIC_ROUNDING) ? 1 : 0;
     p.streaming = static_cast<int>(cfg.streaming);
     p.gaussian = (cfg.flags & SLM_FLAG_GAUSSIAN_DIFFUSE) ? 1 : 0;
+    p.gauss_rclass = gauss_rclass_;
     p.window_sensing = (cfg.flags & SLM_FLAG_WINDOW_SENSING) ? 1 : 0;
     const bool persistent = (cfg.flags & SLM_FLAG_PERSISTENT) != 0;
     p.claim_agents = persistent ? claim_d_ : nullptr;
@@ -412,6 +413,16 @@ class Engine {
     spec[k].params = p;
     launch_set_params(params_d_ + k, make_dev_params(p), stream);
     SLM_HIP(hipGetLastError());
+    if (cfg.flags & SLM_FLAG_GAUSSIAN_DIFFUSE) {
+      // a radius in another class needs the Gaussian pass with another LDS
+      // tile, which captured chunks do not launch: wait for them, drop them
+      const int c = gauss_class_of_specs();
+      if (c != gauss_rclass_) {
+        SLM_HIP(hipStreamSynchronize(stream));
+        drop_graphs();
+        gauss_rclass_ = c;
+      }
+    }
     return SLM_OK;
   }
 
@@ -1117,6 +1128,7 @@ class Engine {
   uint32_t total_agents_ = 0, total_units_ = 0, total_bins_ = 0;
   uint32_t total_tiles_ = 0, max_items_ = 0;
   uint32_t total_gtiles_ = 0;  // Gaussian-pass tiles
+  int gauss_rclass_ = kGaussMaxR;  // Gaussian-pass LDS radius class (gauss_class)
   uint32_t preload_rows_ = 0;
   uint32_t* tile_inst_d_ = nullptr;
   WorkItem* items_d_ = nullptr;
@@ -1229,7 +1241,20 @@ class Engine {
     std::vector<DevParams> dp(spec.size());
     for (size_t k = 0; k < spec.size(); k++) dp[k] = make_dev_params(spec[k].params);
     SLM_HIP(hipMemcpy(params_d_, dp.data(), dp.size() * sizeof(DevParams), hipMemcpyHostToDevice));
+    gauss_rclass_ = gauss_class_of_specs();
     return SLM_OK;
+  }
+
+  // The Gaussian pass's radius class (gauss_class) for the largest radius of
+  // the instances' current parameters.
+  int gauss_class_of_specs() const {
+    int rmax = 1;
+    float w[kGaussMaxR + 1];
+    for (const auto& sp : spec) {
+      const int r = gauss_weights(sp.params.blur_radius, sp.params.blur_sigma, w);
+      rmax = r > rmax ? r : rmax;
+    }
+    return gauss_class(rmax);
   }
 
   // Agent slots, agent arrays, re-sort per-slot scratch and the agent block map.

@@ -36,6 +36,10 @@ struct DevParams {
   float gauss_w[9];
 };
 constexpr int kGaussMaxR = 8;
+// The Gaussian pass's LDS tile is sized for the largest radius of a pool,
+// rounded up to 2, 4 or kGaussMaxR (gauss_class): the presets' radius 2 then
+// needs 37 KiB per bf16 tile instead of 52, and four workgroups fit a CU.
+constexpr int gauss_class(int rmax) { return rmax <= 2 ? 2 : (rmax <= 4 ? 4 : kGaussMaxR); }
 
 // Gaussian weights for blur_radius / blur_sigma; returns the radius, or 0 if
 // they do not describe a Gaussian of radius 1..kGaussMaxR.
@@ -109,6 +113,7 @@ struct Pool {
   int stochastic_rounding;     // bf16 trail-pass stores round stochastically (deterministic hash)
   int streaming;               // trail-pass rows: 1 non-temporal stores, 2 non-temporal loads
   int gaussian;                // trail pass diffuses with the Gaussian, not the 3x3 box
+  int gauss_rclass;            // Gaussian pass LDS sized for radii up to 2, 4 or 8
   int window_sensing;          // one-agent-per-thread workgroups sense through an LDS window
   uint32_t* claim_agents;      // persistent agent / trail passes: kClaimWords counters each
   uint32_t* claim_trail;       // (nullptr: one workgroup per logical workgroup)

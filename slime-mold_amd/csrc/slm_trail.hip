@@ -608,15 +608,15 @@ void launch_trail_t(const Pool& pool, const void* in, void* out, hipStream_t str
 // tile and its halo into registers, and the vertical pass from them.  Both
 // passes start from the product with the first tap's weight and add the others
 // in ascending tap order with one fused multiply-add each, so the result is
-// the oracle's bit for bit.  LDS is sized for the largest radius, so a
-// captured graph stays valid when slm_set_params changes the radius.
+// the oracle's bit for bit.  LDS is sized for the radius class RM of the pool
+// (gauss_class of its largest radius); the engine recaptures its graphs when
+// slm_set_params moves a radius to another class.
 struct GaussArgs {
   const void* in;
   void* out;
   uint32_t count;  // instances
 };
 
-constexpr int kGaussCols = static_cast<int>(kGaussTW) + 2 * kGaussMaxR;
 // tile rows of storage S (Codec<S>: the stochastic-rounding format is 16-bit)
 template <int S>
 constexpr int gauss_rows_of() {
@@ -658,8 +658,8 @@ __device__ inline float gauss_stage(uint32_t raw, const DevParams& p, float sat)
 // as chunks (plus single halo cells), and the results are staged in LDS and
 // stored as chunks.  Other tiles load and store one cell per lane.  Both give
 // the same bits.
-template <int S, int R, bool kSat>
-__device__ inline void gauss_tile(float (*s_t)[kGaussCols],
+template <int S, int R, bool kSat, int kCols, bool kPipe>
+__device__ inline void gauss_tile(float (*s_t)[kCols],
                                   const typename Codec<S>::elem* src, typename Codec<S>::elem* dst,
                                   uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
                                   const DevParams& p, const float* wts) {
@@ -748,12 +748,41 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
   // the horizontal pass of the thread's own column, every row of the tile
   // and its halo, stays in registers for the vertical pass
   float h[rows];
+  if constexpr (kPipe) {
+    // one ds_read_b32 per tap at an immediate offset from a single base,
+    // one row ahead of the arithmetic: paired into ds_read2_b32 (8-bit
+    // offsets, which cannot span a 1 KiB row) the reads cost two address
+    // adds per row, 7% of this VALU-bound pass.  Relaxed atomic loads are
+    // never paired; nothing else orders them.
+    auto ld = [&](int r, int i) {
+      return __uint_as_float(__atomic_load_n(reinterpret_cast<uint32_t*>(&s_t[r][c + i]),
+                                             __ATOMIC_RELAXED));
+    };
+    float cur[2 * R + 1], nxt[2 * R + 1];
 #pragma unroll
-  for (int r = 0; r < rows; r++) {
-    float sum = w[R] * s_t[r][c];
+    for (int i = 0; i <= 2 * R; i++) cur[i] = ld(0, i);
 #pragma unroll
-    for (int i = 1; i <= 2 * R; i++) sum = __builtin_fmaf(w[i < R ? R - i : i - R], s_t[r][c + i], sum);
-    h[r] = sum;
+    for (int r = 0; r < rows; r++) {
+      if (r + 1 < rows) {
+#pragma unroll
+        for (int i = 0; i <= 2 * R; i++) nxt[i] = ld(r + 1, i);
+      }
+      float sum = w[R] * cur[0];
+#pragma unroll
+      for (int i = 1; i <= 2 * R; i++) sum = __builtin_fmaf(w[i < R ? R - i : i - R], cur[i], sum);
+      h[r] = sum;
+#pragma unroll
+      for (int i = 0; i <= 2 * R; i++) cur[i] = nxt[i];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < rows; r++) {
+      float sum = w[R] * s_t[r][c];
+#pragma unroll
+      for (int i = 1; i <= 2 * R; i++)
+        sum = __builtin_fmaf(w[i < R ? R - i : i - R], s_t[r][c + i], sum);
+      h[r] = sum;
+    }
   }
   const uint32_t x = x0 + static_cast<uint32_t>(c);
   // hashed cell index of (x, y) for stochastic rounding, stepped a row at a time
@@ -805,11 +834,12 @@ __device__ inline void gauss_tile(float (*s_t)[kGaussCols],
   }
 }
 
-template <int S>
+template <int S, int RM>
 __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
                                                          const Instance* __restrict__ insts,
                                                          const DevParams* __restrict__ params) {
-  __shared__ __align__(16) float s_t[gauss_rows_of<S>() + 2 * kGaussMaxR][kGaussCols];
+  constexpr int kCols = static_cast<int>(kGaussTW) + 2 * RM;
+  __shared__ __align__(16) float s_t[gauss_rows_of<S>() + 2 * RM][kCols];
   using elem = typename Codec<S>::elem;
   const uint32_t t = blockIdx.x;
   uint32_t lo = 0, hi = a.count - 1u;  // last instance whose tiles start at or before t
@@ -827,18 +857,27 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
   const elem* src = static_cast<const elem*>(a.in) + in.trail_off;
   elem* dst = static_cast<elem*>(a.out) + in.trail_off;
   const float* wts = params[lo].gauss_w;
-  // 1..kGaussMaxR (the host checks it); any other value writes nothing
+  // 1..RM (the host sizes RM for the pool's largest radius); any other value
+  // writes nothing
   auto run = [&](auto sat) {
     constexpr bool kSat = decltype(sat)::value;
+    auto tile = [&](auto r) {
+      constexpr int R = decltype(r)::value;
+      // the pipelined horizontal reads in the 16-bit radius-2 class only
+      // (with every radius in one kernel they raised its VGPRs to 214)
+      constexpr bool kPipe = RM == 2 && sizeof(elem) == 2;
+      if constexpr (R <= RM)
+        gauss_tile<S, R, kSat, kCols, kPipe>(s_t, src, dst, in.width, in.height, x0, y0, p, wts);
+    };
     switch (p.gauss_r) {
-      case 1: gauss_tile<S, 1, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-      case 2: gauss_tile<S, 2, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-      case 3: gauss_tile<S, 3, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-      case 4: gauss_tile<S, 4, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-      case 5: gauss_tile<S, 5, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-      case 6: gauss_tile<S, 6, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-      case 7: gauss_tile<S, 7, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
-      case 8: gauss_tile<S, 8, kSat>(s_t, src, dst, in.width, in.height, x0, y0, p, wts); break;
+      case 1: tile(std::integral_constant<int, 1>{}); break;
+      case 2: tile(std::integral_constant<int, 2>{}); break;
+      case 3: tile(std::integral_constant<int, 3>{}); break;
+      case 4: tile(std::integral_constant<int, 4>{}); break;
+      case 5: tile(std::integral_constant<int, 5>{}); break;
+      case 6: tile(std::integral_constant<int, 6>{}); break;
+      case 7: tile(std::integral_constant<int, 7>{}); break;
+      case 8: tile(std::integral_constant<int, 8>{}); break;
       default: break;
     }
   };
@@ -852,7 +891,22 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
 template <int S>
 void launch_gauss_t(const Pool& pool, const void* in, void* out, hipStream_t stream) {
   GaussArgs a{in, out, pool.count};
-  gauss_kernel<S><<<pool.total_gtiles, kGaussTW, 0, stream>>>(a, pool.inst, pool.params);
+  // f32 tiles keep the largest LDS tile: at radius 2 the smaller one (seven
+  // workgroups per CU instead of four) measured 3-4% slower on the HBM-bound
+  // f32 pass, while the VALU-bound 16-bit pass gains 11% from a fourth
+  auto go = [&](auto* kernel) {
+    kernel<<<pool.total_gtiles, kGaussTW, 0, stream>>>(a, pool.inst, pool.params);
+  };
+  if constexpr (sizeof(typename Codec<S>::elem) == 4) {
+    go(gauss_kernel<S, kGaussMaxR>);
+  } else {
+    if (pool.gauss_rclass <= 2)
+      go(gauss_kernel<S, 2>);
+    else if (pool.gauss_rclass <= 4)
+      go(gauss_kernel<S, 4>);
+    else
+      go(gauss_kernel<S, kGaussMaxR>);
+  }
 }
 
 }  // namespace

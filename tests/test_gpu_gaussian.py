@@ -103,19 +103,53 @@ def test_gaussian_batch_radius_per_instance(slm, orc, groups):
 
 
 def test_gaussian_radius_change_under_captured_graphs(slm, orc):
+    """Radius changes between captured chunks, within a radius class (the
+    chunks replay) and across classes 2 / 4 / 8 (the pass's LDS tile changes
+    and the chunks are recaptured), back and forth."""
     s2 = dataclasses.replace(preset(slm, "Default"), blur_radius=2.0, blur_sigma=1.0)
-    s8 = dataclasses.replace(s2, blur_radius=8.0, blur_sigma=4.0)
-    print("[gauss radius change] 400x200 n=20000 chunk=5", flush=True)
+    seq = [(s2, 10),
+           (dataclasses.replace(s2, blur_radius=1.0, blur_sigma=0.7), 5),   # same class
+           (dataclasses.replace(s2, blur_radius=8.0, blur_sigma=4.0), 10),  # class 8
+           (dataclasses.replace(s2, blur_radius=3.0, blur_sigma=1.5), 5),   # class 4
+           (s2, 10)]                                                          # class 2
+    print("[gauss radius change] 400x200 n=20000 chunk=5 radii 2,1,8,3,2", flush=True)
     with slm.Simulation(400, 200, 20000, seed=5, params=s2, graph_chunk=5, gaussian=True) as sim:
         a, t = sim.read_agents(), sim.read_trail()
-        sim.step(10)  # graphs captured with radius 2
-        sim.set_params(s8)
-        sim.step(10)  # replayed with radius 8
+        for s, n in seq:
+            sim.set_params(s)
+            sim.step(n)
         ga, gt = sim.read_agents(), sim.read_trail()
-    orc.step(a, t, s2, 10, gaussian=True)
-    orc.step(a, t, s8, 10, gaussian=True)
+    for s, n in seq:
+        orc.step(a, t, s, n, gaussian=True)
     assert_bitwise(ga, a, "agents")
     assert_bitwise(gt, t, "trail")
+
+
+def test_gaussian_batch_radius_class_change(slm, orc):
+    """One instance of a batch moves its radius to another class while the
+    others keep theirs: the pass's tile follows the largest radius."""
+    base = dataclasses.replace(preset(slm, "Default"), blur_radius=2.0, blur_sigma=1.0)
+    specs = [slm.InstanceSpec(w, h, n, seed=11 + k, params=base)
+             for k, (w, h, n) in enumerate([(256, 64, 3000), (130, 70, 2000), (300, 40, 4000)])]
+    print("[gauss batch radius class] 3 instances, chunk 4, instance 1 radius 2 -> 6 -> 2",
+          flush=True)
+    with slm.Batch(specs, graph_chunk=4, gaussian=True, storage="bf16",
+                   stochastic_rounding=True) as b:
+        starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+        plan = [(None, 8), (dataclasses.replace(base, blur_radius=6.0, blur_sigma=2.5), 8),
+                (base, 8)]
+        for s, n in plan:
+            if s is not None:
+                b.set_params(1, s)
+            b.step(n)
+        got = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
+    for k in range(len(specs)):
+        a, t = starts[k]
+        for s, n in plan:
+            orc.step(a, t, s if (s is not None and k == 1) else base, n, storage="bf16sr",
+                     gaussian=True)
+        assert_bitwise(got[k][0], a, f"instance {k} agents")
+        assert_bitwise(got[k][1], t, f"instance {k} trail")
 
 
 def test_gaussian_params_are_checked(slm):
