@@ -199,7 +199,11 @@ int slm_destroy(slm_sim* sim);
 
 /* Replace the parameters for subsequent steps; agents and trail persist.
  * Replaces update_settings / the preset switch (src/main.rs:83-99, :815-832).
- * Captured graphs stay valid: kernels read parameters from device memory. */
+ * Captured graphs stay valid: kernels read parameters from device memory.
+ * One exception, with SLM_FLAG_GAUSSIAN_DIFFUSE: a 16-bit pool's Gaussian pass
+ * sizes its LDS tile for the largest radius class (R <= 2, <= 4, <= 8) of its
+ * instances, so a change of class waits for the handle's stream and drops
+ * the captured graphs, which the next slm_step recaptures. */
 int slm_set_params(slm_sim* sim, const slm_params* params);
 int slm_get_params(const slm_sim* sim, slm_params* out);
 
