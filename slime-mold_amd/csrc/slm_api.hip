@@ -305,6 +305,7 @@ class Engine {
   int create(const EngineConfig& c, std::vector<InstanceSpec> s) {
     cfg = c;
     spec = std::move(s);
+    gauss_rclass_ = gauss_class_of_specs();  // the Gaussian tiles depend on it
     reassigns.assign(spec.size(), 0);
     SLM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     SLM_TRY(alloc_const());
@@ -415,12 +416,15 @@ class Engine {
     SLM_HIP(hipGetLastError());
     if (cfg.flags & SLM_FLAG_GAUSSIAN_DIFFUSE) {
       // a radius in another class needs the Gaussian pass with another LDS
-      // tile, which captured chunks do not launch: wait for them, drop them
+      // tile (and, for 16-bit storage, another tile height, so another tile
+      // plan), which captured chunks do not launch: wait for them, drop them
       const int c = gauss_class_of_specs();
       if (c != gauss_rclass_) {
         SLM_HIP(hipStreamSynchronize(stream));
         drop_graphs();
+        const bool replan = gauss_th(cfg.storage, c) != gauss_th(cfg.storage, gauss_rclass_);
         gauss_rclass_ = c;
+        if (replan) return relayout(layout_trail_tables());
       }
     }
     return SLM_OK;
@@ -1241,7 +1245,6 @@ class Engine {
     std::vector<DevParams> dp(spec.size());
     for (size_t k = 0; k < spec.size(); k++) dp[k] = make_dev_params(spec[k].params);
     SLM_HIP(hipMemcpy(params_d_, dp.data(), dp.size() * sizeof(DevParams), hipMemcpyHostToDevice));
-    gauss_rclass_ = gauss_class_of_specs();
     return SLM_OK;
   }
 
@@ -1361,7 +1364,7 @@ class Engine {
       tiles += in.tiles;
       in.gtx = (in.width + kGaussTW - 1u) / kGaussTW;
       in.gtile_off = static_cast<uint32_t>(gtiles);
-      const uint32_t th = gauss_th(cfg.storage);
+      const uint32_t th = gauss_th(cfg.storage, gauss_rclass_);
       gtiles += static_cast<uint64_t>(in.gtx) * ((in.height + th - 1u) / th);
     }
     if (gtiles >= (1ull << 31)) return fail(SLM_ERR_INVALID_ARGUMENT, "too many trail tiles");
@@ -1576,7 +1579,7 @@ class Engine {
       G.units = z.unit_off + z.units - a.unit_off;
       G.trail_base = a.trail_off;
       G.gtile_base = a.gtile_off;
-      const uint32_t th = gauss_th(cfg.storage);
+      const uint32_t th = gauss_th(cfg.storage, gauss_rclass_);
       G.gtiles = z.gtile_off + z.gtx * ((z.height + th - 1u) / th) - a.gtile_off;
       std::vector<Instance> ins(inst.begin() + G.k0, inst.begin() + G.k1);
       bool first = true;

@@ -120,12 +120,16 @@ struct Pool {
   uint32_t total_gtiles;       // Gaussian-pass tiles of the pool
 };
 
-// Gaussian-pass tiles: kGaussTW columns x gauss_th(storage) rows per
-// workgroup.  Taller tiles re-read and re-filter fewer halo rows but hold more
-// LDS: 16 rows for f32 (HBM-bound; measured 72.6% of peak against 70.5% with
-// 32), 32 rows for the 16-bit formats (VALU-bound; 45.1% against 44.0%).
+// Gaussian-pass tiles: kGaussTW columns x gauss_th(storage, radius class)
+// rows per workgroup.  Taller tiles re-read and re-filter fewer halo rows but
+// hold more LDS: 16 rows for f32 (HBM-bound; measured 72.6% of peak against
+// 70.5% with 32); for the 16-bit formats (VALU-bound) 16 rows up to radius 4
+// (five to seven workgroups per CU: 230.8 against 241.1 us at radius 2) and
+// 32 rows up to radius 8 (597.8 against 686.6 us at radius 8, 16384^2 bf16).
 constexpr uint32_t kGaussTW = 256;
-constexpr uint32_t gauss_th(int storage) { return storage == SLM_STORAGE_F32 ? 16u : 32u; }
+constexpr uint32_t gauss_th(int storage, int rclass) {
+  return storage == SLM_STORAGE_F32 || rclass <= 4 ? 16u : 32u;
+}
 
 constexpr uint32_t kTileLog2 = 6;  // sensing tiles of 64 x 64 cells
 

@@ -617,11 +617,12 @@ struct GaussArgs {
   uint32_t count;  // instances
 };
 
-// tile rows of storage S (Codec<S>: the stochastic-rounding format is 16-bit)
-template <int S>
+// tile rows of storage S in radius class RM (Codec<S>: the stochastic-rounding
+// format is 16-bit)
+template <int S, int RM>
 constexpr int gauss_rows_of() {
-  return static_cast<int>(sizeof(typename Codec<S>::elem) == 4 ? gauss_th(SLM_STORAGE_F32)
-                                                                : gauss_th(SLM_STORAGE_BF16));
+  return static_cast<int>(sizeof(typename Codec<S>::elem) == 4 ? gauss_th(SLM_STORAGE_F32, RM)
+                                                                : gauss_th(SLM_STORAGE_BF16, RM));
 }
 
 // v mod n for |v| < 2^25 and 1 <= n <= 2^24 (grid sides are at most 2^24).
@@ -658,13 +659,12 @@ __device__ inline float gauss_stage(uint32_t raw, const DevParams& p, float sat)
 // as chunks (plus single halo cells), and the results are staged in LDS and
 // stored as chunks.  Other tiles load and store one cell per lane.  Both give
 // the same bits.
-template <int S, int R, bool kSat, int kCols, bool kPipe>
+template <int S, int R, bool kSat, int kCols, bool kPipe, int kGaussTH>
 __device__ inline void gauss_tile(float (*s_t)[kCols],
                                   const typename Codec<S>::elem* src, typename Codec<S>::elem* dst,
                                   uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
                                   const DevParams& p, const float* wts) {
   using elem = typename Codec<S>::elem;
-  constexpr int kGaussTH = gauss_rows_of<S>();
   constexpr int rows = kGaussTH + 2 * R;
   constexpr int cols = static_cast<int>(kGaussTW) + 2 * R;
   constexpr int V = static_cast<int>(16 / sizeof(elem));  // cells per 16-byte chunk
@@ -839,7 +839,8 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
                                                          const Instance* __restrict__ insts,
                                                          const DevParams* __restrict__ params) {
   constexpr int kCols = static_cast<int>(kGaussTW) + 2 * RM;
-  __shared__ __align__(16) float s_t[gauss_rows_of<S>() + 2 * RM][kCols];
+  constexpr int kTH = gauss_rows_of<S, RM>();
+  __shared__ __align__(16) float s_t[kTH + 2 * RM][kCols];
   using elem = typename Codec<S>::elem;
   const uint32_t t = blockIdx.x;
   uint32_t lo = 0, hi = a.count - 1u;  // last instance whose tiles start at or before t
@@ -853,7 +854,7 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
   const Instance in = insts[lo];
   const DevParams p = params[lo];
   const uint32_t lt = t - in.gtile_off;
-  const uint32_t x0 = (lt % in.gtx) * kGaussTW, y0 = (lt / in.gtx) * gauss_rows_of<S>();
+  const uint32_t x0 = (lt % in.gtx) * kGaussTW, y0 = (lt / in.gtx) * kTH;
   const elem* src = static_cast<const elem*>(a.in) + in.trail_off;
   elem* dst = static_cast<elem*>(a.out) + in.trail_off;
   const float* wts = params[lo].gauss_w;
@@ -867,7 +868,8 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
       // (with every radius in one kernel they raised its VGPRs to 214)
       constexpr bool kPipe = RM == 2 && sizeof(elem) == 2;
       if constexpr (R <= RM)
-        gauss_tile<S, R, kSat, kCols, kPipe>(s_t, src, dst, in.width, in.height, x0, y0, p, wts);
+        gauss_tile<S, R, kSat, kCols, kPipe, kTH>(s_t, src, dst, in.width, in.height, x0, y0, p,
+                                                  wts);
     };
     switch (p.gauss_r) {
       case 1: tile(std::integral_constant<int, 1>{}); break;

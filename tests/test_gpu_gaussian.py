@@ -102,25 +102,29 @@ def test_gaussian_batch_radius_per_instance(slm, orc, groups):
         assert_bitwise(got[k][1], t, f"instance {k} trail")
 
 
-def test_gaussian_radius_change_under_captured_graphs(slm, orc):
+@pytest.mark.parametrize("storage", ["f32", "bf16sr"])
+def test_gaussian_radius_change_under_captured_graphs(slm, orc, storage):
     """Radius changes between captured chunks, within a radius class (the
     chunks replay) and across classes 2 / 4 / 8 (the pass's LDS tile changes
-    and the chunks are recaptured), back and forth."""
+    and the chunks are recaptured; in 16-bit storage the tile height and the
+    tile plan change too), back and forth."""
     s2 = dataclasses.replace(preset(slm, "Default"), blur_radius=2.0, blur_sigma=1.0)
     seq = [(s2, 10),
            (dataclasses.replace(s2, blur_radius=1.0, blur_sigma=0.7), 5),   # same class
            (dataclasses.replace(s2, blur_radius=8.0, blur_sigma=4.0), 10),  # class 8
            (dataclasses.replace(s2, blur_radius=3.0, blur_sigma=1.5), 5),   # class 4
            (s2, 10)]                                                          # class 2
-    print("[gauss radius change] 400x200 n=20000 chunk=5 radii 2,1,8,3,2", flush=True)
-    with slm.Simulation(400, 200, 20000, seed=5, params=s2, graph_chunk=5, gaussian=True) as sim:
+    print(f"[gauss radius change] 400x200 n=20000 chunk=5 radii 2,1,8,3,2 {storage}", flush=True)
+    sr = storage == "bf16sr"
+    with slm.Simulation(400, 200, 20000, seed=5, params=s2, graph_chunk=5, gaussian=True,
+                        storage="bf16" if sr else storage, stochastic_rounding=sr) as sim:
         a, t = sim.read_agents(), sim.read_trail()
         for s, n in seq:
             sim.set_params(s)
             sim.step(n)
         ga, gt = sim.read_agents(), sim.read_trail()
     for s, n in seq:
-        orc.step(a, t, s, n, gaussian=True)
+        orc.step(a, t, s, n, storage=storage, gaussian=True)
     assert_bitwise(ga, a, "agents")
     assert_bitwise(gt, t, "trail")
 
