@@ -201,9 +201,10 @@ int slm_destroy(slm_sim* sim);
  * Replaces update_settings / the preset switch (src/main.rs:83-99, :815-832).
  * Captured graphs stay valid: kernels read parameters from device memory.
  * One exception, with SLM_FLAG_GAUSSIAN_DIFFUSE: a 16-bit pool's Gaussian pass
- * sizes its LDS tile for the largest radius class (R <= 2, <= 4, <= 8) of its
- * instances, so a change of class waits for the handle's stream and drops
- * the captured graphs, which the next slm_step recaptures. */
+ * sizes its LDS tile (and its tile height) for the largest radius class
+ * (R <= 2, <= 4, <= 8) of its instances, so a change of class waits for the
+ * handle's stream, drops the captured graphs (the next slm_step recaptures
+ * them) and, when the tile height changes, rebuilds the tile plan. */
 int slm_set_params(slm_sim* sim, const slm_params* params);
 int slm_get_params(const slm_sim* sim, slm_params* out);
 
