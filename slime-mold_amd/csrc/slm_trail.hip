@@ -864,9 +864,10 @@ __global__ __launch_bounds__(kGaussTW) void gauss_kernel(GaussArgs a,
     constexpr bool kSat = decltype(sat)::value;
     auto tile = [&](auto r) {
       constexpr int R = decltype(r)::value;
-      // the pipelined horizontal reads in the 16-bit radius-2 class only
-      // (with every radius in one kernel they raised its VGPRs to 214)
-      constexpr bool kPipe = RM == 2 && sizeof(elem) == 2;
+      // the pipelined horizontal reads in the 16-bit radius classes 2 and 4
+      // (radius 4: 333 -> 324 us on 16384^2 bf16); with every radius in one
+      // kernel they had raised its VGPRs to 214
+      constexpr bool kPipe = RM <= 4 && sizeof(elem) == 2;
       if constexpr (R <= RM)
         gauss_tile<S, R, kSat, kCols, kPipe, kTH>(s_t, src, dst, in.width, in.height, x0, y0, p,
                                                   wts);
