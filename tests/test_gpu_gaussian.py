@@ -129,16 +129,19 @@ def test_gaussian_radius_change_under_captured_graphs(slm, orc, storage):
     assert_bitwise(gt, t, "trail")
 
 
-def test_gaussian_batch_radius_class_change(slm, orc):
+@pytest.mark.parametrize("groups", [1, 2])
+def test_gaussian_batch_radius_class_change(slm, orc, groups):
     """One instance of a batch moves its radius to another class while the
-    others keep theirs: the pass's tile follows the largest radius."""
+    others keep theirs: the pass's tile follows the largest radius (also with
+    the instances in two groups on their own streams)."""
     base = dataclasses.replace(preset(slm, "Default"), blur_radius=2.0, blur_sigma=1.0)
     specs = [slm.InstanceSpec(w, h, n, seed=11 + k, params=base)
              for k, (w, h, n) in enumerate([(256, 64, 3000), (130, 70, 2000), (300, 40, 4000)])]
-    print("[gauss batch radius class] 3 instances, chunk 4, instance 1 radius 2 -> 6 -> 2",
-          flush=True)
+    print(f"[gauss batch radius class] 3 instances, chunk 4, instance 1 radius 2 -> 6 -> 2, "
+          f"{groups} groups", flush=True)
     with slm.Batch(specs, graph_chunk=4, gaussian=True, storage="bf16",
                    stochastic_rounding=True) as b:
+        b.set_tuning(instance_groups=groups)
         starts = [(b.read_agents(k), b.read_trail(k)) for k in range(len(specs))]
         plan = [(None, 8), (dataclasses.replace(base, blur_radius=6.0, blur_sigma=2.5), 8),
                 (base, 8)]
