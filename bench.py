@@ -69,9 +69,9 @@ WRITE_SIZE_SCALE = 1.0
 MEASURED_TRAFFIC = {
     # (kernel, W, H, N, storage, tuning, steps, warmup): (FETCH KiB, WRITE KiB)
     ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 1024, 3, 0), 20, 5):
-        (181323, 274830),
+        (181524, 278540),
     ("agents_kernel", 4096, 4096, 1 << 24, "f32", (15, 3, 16, 1024, 3, 0), 2000, 100):
-        (169089, 218975),
+        (169235, 219605),
 }
 
 
